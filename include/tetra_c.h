@@ -1,0 +1,139 @@
+/*
+ * tetra_c.h -- C ABI of libtetra_hip.so: TetraGPU's relation builders and
+ * critical-point classifier on one MI355X (gfx950).
+ *
+ * Every function returns TG_OK (0) or a negative TG_ERR_* status; the
+ * message of the last failure on the calling thread is tg_last_error().
+ * Work is enqueued on the mesh's HIP stream and runs asynchronously;
+ * functions that hand results to the host synchronise that stream.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *  - make_TV_for_GPU / make_VE_for_GPU / make_VF_for_GPU
+ *      (include/cuda_extraction.h:11-23): tg_mesh_create uploads TV once;
+ *      VE and VF are built on the device, not flattened on the host.
+ *  - make_{EV,TF,TE,FV,FE}_GPU_return / make_{...}_GPU
+ *      (include/cuda_extraction.h:27-114) and make_VV_GPU_return /
+ *      make_VV_GPU (:119-127), make_VT_GPU_return (:137-138):
+ *      tg_build(flags) + tg_relation_device / tg_relation_copy.
+ *  - get_approx_max_VV (include/cuda_extraction.h:118): tg_approx_max_vv.
+ *  - the critPoints kernel and its driver (src/alg/critPoints.cu:49-61,
+ *      1092-1269): tg_critpoints / tg_vv_critpoints + tg_classes_copy,
+ *      same classes[3*nPoints] layout.
+ *  - the relation flag bits of runtime_arguments
+ *      (include/datatypes.h.in:158-178): TG_REL_* are the same bit
+ *      positions, TG_FLAG(rel) the mask.
+ */
+#ifndef TETRA_C_H
+#define TETRA_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define TG_OK             0
+#define TG_ERR_ARG       -1
+#define TG_ERR_HIP       -2
+#define TG_ERR_ALLOC     -3
+#define TG_ERR_MESH      -4  /* vertex id out of range or repeated in a cell */
+#define TG_ERR_CAPACITY  -5  /* a vertex exceeds the supported degree        */
+#define TG_ERR_STATE     -6  /* relation not built / inputs missing         */
+#define TG_ERR_NO_DEVICE -7
+
+/* relation ids = bit positions of the reference's arg_flags */
+#define TG_REL_TE 1
+#define TG_REL_EV 2
+#define TG_REL_ET 3
+#define TG_REL_TF 4
+#define TG_REL_FV 5
+#define TG_REL_FE 6
+#define TG_REL_FT 7
+#define TG_REL_EF 8
+#define TG_REL_VT 9
+#define TG_REL_TT 10
+#define TG_REL_FF 11
+#define TG_REL_EE 12
+#define TG_REL_VV 13
+/* always-initialised relations of the reference (not flag-selectable) */
+#define TG_REL_VE 14
+#define TG_REL_VF 15
+#define TG_FLAG(rel) (1u << (rel))
+#define TG_ALL_RELATIONS 0x3FFEu /* bits 1..13 */
+
+typedef struct tg_mesh tg_mesh;
+
+/* Select and warm up a device (hipSetDevice + context creation). */
+int tg_init(int device);
+const char *tg_last_error(void);
+int tg_device_count(int *count);
+
+/* Create a mesh from int32 cells [n_cells][4] and optional per-vertex
+ * scalars (float32 when scalar_f64 == 0, else float64; NULL for none).
+ * inputs_on_device: the pointers are device pointers (copied on device).
+ * stream: hipStream_t to run on, or NULL for a stream owned by the mesh.
+ * Connectivity is validated once here (ids in range, no repeated vertex in
+ * a cell).  Mirrors make_TV_for_GPU (include/cuda_extraction.h:11). */
+int tg_mesh_create(tg_mesh **out, const int32_t *cells, int64_t n_cells,
+                   int64_t n_points, const void *scalars, int scalar_f64,
+                   int inputs_on_device, void *stream);
+int tg_mesh_destroy(tg_mesh *m);
+/* Replace the scalar field (resets critical-point results). */
+int tg_mesh_set_scalars(tg_mesh *m, const void *scalars, int scalar_f64,
+                        int on_device);
+int tg_mesh_stream(tg_mesh *m, void **stream);
+int tg_mesh_counts(tg_mesh *m, int64_t *n_points, int64_t *n_cells,
+                   int64_t *n_edges, int64_t *n_faces);
+
+/* Kernel shape of the VV + critical point pipeline: neighbours held per lane
+ * (32 or 64) and lanes per workgroup (64, 128 or 256).  Chosen by the
+ * autotuner per size class; any supported value gives identical results. */
+int tg_mesh_set_config(tg_mesh *m, int32_t lane_cap, int32_t lane_block);
+
+/* Build the relations whose TG_FLAG bits are set, plus what they need.
+ * Bit 0 (validate) is ignored.  Asynchronous. */
+int tg_build(tg_mesh *m, uint32_t flags);
+/* VV (if not built) and classes[3V]. Asynchronous. */
+int tg_critpoints(tg_mesh *m);
+/* The measured hot path: VV CSR + classes recomputed from TV and scalars
+ * resident in HBM, no host synchronisation, no allocation after the first
+ * call on a mesh (graph-capturable). */
+int tg_vv_critpoints(tg_mesh *m);
+/* Wait for the mesh's stream; reports device-side capacity errors. */
+int tg_sync(tg_mesh *m);
+
+/* Shape of a built relation: rows, entries, int32 words per entry, and
+ * fixed = 1 for fixed-width relations (no offsets). Synchronises. */
+int tg_relation_shape(tg_mesh *m, int32_t rel, int64_t *rows, int64_t *nnz,
+                      int32_t *width, int32_t *fixed);
+/* Copy a built relation to host memory: offsets[rows+1] (ignored for fixed
+ * relations, may be NULL) and values[nnz*width]. Synchronises. */
+int tg_relation_copy(tg_mesh *m, int32_t rel, int64_t *offsets,
+                     int32_t *values);
+/* Device pointers of a built relation (owned by the mesh). offset_bytes is
+ * 4 or 8 (0 for fixed relations). */
+int tg_relation_device(tg_mesh *m, int32_t rel, const void **d_offsets,
+                       int32_t *offset_bytes, const int32_t **d_values);
+
+/* classes[3v..3v+2] = (lower link components, upper link components,
+ * class 1 minimum / 2 maximum / 3 regular / 4 saddle). Synchronises. */
+int tg_classes_copy(tg_mesh *m, uint32_t *classes3);
+int tg_classes_device(tg_mesh *m, const uint32_t **d_classes3);
+/* Six-way labels numbered as TTK's CriticalType (0 minimum, 1 1-saddle,
+ * 2 2-saddle, 3 maximum, 4 multi-saddle, 5 regular). Synchronises. */
+int tg_labels6_copy(tg_mesh *m, uint8_t *labels);
+
+/* round_up_32(3 * largest number of cells on one vertex), the reference's
+ * padded VV width (src/cuda_extraction.cu:1154-1178), computed on the
+ * device for a built mesh. */
+int tg_approx_max_vv(tg_mesh *m, int32_t *max_vv);
+
+/* Free a device pointer handed out by the library to the caller. */
+int tg_device_free(void *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

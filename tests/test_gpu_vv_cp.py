@@ -1,0 +1,133 @@
+"""GPU parity of the VV relation and critical-point classes against the CPU
+oracle (restatement of src/cpu_extraction.cpp:259-289 and
+src/alg/critPoints.cu:49-338).  Bar: bit-exact classes[3V]; VV rows equal as
+sets (the reference leaves the order within a row unspecified; the device
+emits ascending rows, so rows are compared after sorting the oracle's)."""
+import numpy as np
+import pytest
+
+from oracle import binding as O
+
+pytestmark = pytest.mark.gpu
+
+
+def run_device(t, cells, n_points, scalars, cfg=None, twice=False):
+    with t.Mesh(cells, n_points, scalars) as m:
+        if cfg:
+            m.set_config(*cfg)
+        m.vv_critpoints()
+        if twice:
+            m.vv_critpoints()
+        off, vals = m.relation("VV")
+        return off, vals[:, 0].copy(), m.classes()
+
+
+def check(t, cells, n_points, scalars, cfg=None, twice=False):
+    print(f"case V={n_points} T={len(cells)} cfg={cfg}", flush=True)
+    off, cols, cls = run_device(t, cells, n_points, scalars, cfg, twice)
+    VV = O.vv(cells, n_points).sorted_rows()
+    assert np.array_equal(off, VV.off), "VV offsets differ"
+    assert np.array_equal(cols, VV.val[:, 0]), "VV rows differ"
+    ref = O.critpoints(O.vv(cells, n_points), np.asarray(scalars, np.float64), n_points)
+    bad = np.nonzero((cls != ref).any(axis=1))[0]
+    assert bad.size == 0, f"{bad.size} vertices differ, first {bad[:5]}: {cls[bad[:5]]} vs {ref[bad[:5]]}"
+    return cls
+
+
+def test_config1_kuhn24(gpu):
+    m = gpu.kuhn(24, seed=1234)
+    cls = check(gpu, m.cells, m.n_points, m.scalars)
+    # uniform noise: every class occurs
+    assert set(np.unique(cls[:, 2])) == {1, 2, 3, 4}
+
+
+@pytest.mark.parametrize("cfg", [(32, 64), (32, 128), (32, 256), (64, 64), (64, 128)])
+def test_lane_shapes_identical(gpu, cfg):
+    m = gpu.kuhn(12, seed=7)
+    check(gpu, m.cells, m.n_points, m.scalars, cfg=cfg)
+
+
+@pytest.mark.parametrize("order", ["path", "shuffled"])
+def test_unsorted_cells(gpu, order):
+    m = gpu.kuhn(10, seed=3, order=order)
+    check(gpu, m.cells, m.n_points, m.scalars)
+
+
+def test_irregular_hubs_over_64_neighbours(gpu):
+    # hubs push some vertex degrees past 64: serial kernels and the big fix path
+    m = gpu.kuhn(12, seed=11, split_fraction=0.15, n_hubs=6, hub_splits=60, scalar="sines")
+    VV = O.vv(m.cells, m.n_points)
+    assert np.diff(VV.off).max() > 64
+    check(gpu, m.cells, m.n_points, m.scalars)
+    check(gpu, m.cells, m.n_points, m.scalars, cfg=(64, 128))
+
+
+def test_all_ties_constant_field(gpu):
+    m = gpu.kuhn(8, seed=5, scalar="constant")
+    check(gpu, m.cells, m.n_points, m.scalars)
+
+
+def test_linear_field_one_min_one_max(gpu):
+    m = gpu.kuhn(9, permute_vertices=False, permute_cells=True, jitter=False, scalar="linear")
+    cls = check(gpu, m.cells, m.n_points, m.scalars)
+    assert (cls[:, 2] == 1).sum() == 1 and (cls[:, 2] == 2).sum() == 1
+    assert (cls[:, 2] == 3).sum() == m.n_points - 2
+
+
+def test_gaussian_field_f64_scalars(gpu):
+    m = gpu.kuhn(14, seed=21, scalar="gaussians")
+    check(gpu, m.cells, m.n_points, m.scalars.astype(np.float64) * (1 + 1e-9))
+
+
+def test_nan_scalars(gpu):
+    m = gpu.kuhn(6, seed=2)
+    s = m.scalars.copy()
+    s[::7] = np.nan
+    check(gpu, m.cells, m.n_points, s)
+
+
+def test_isolated_and_tiny(gpu):
+    # one tet plus three vertices no cell references: (0, 0, 4)
+    cells = np.array([[0, 2, 4, 5]], np.int32)
+    cls = check(gpu, cells, 8, np.arange(8, dtype=np.float32))
+    assert cls[1].tolist() == [0, 0, 4] and cls[3].tolist() == [0, 0, 4]
+
+
+def test_empty_mesh(gpu):
+    off, cols, cls = run_device(gpu, np.zeros((0, 4), np.int32), 5, np.zeros(5, np.float32))
+    assert off.tolist() == [0] * 6 and cols.size == 0
+    assert cls.tolist() == [[0, 0, 4]] * 5
+
+
+def test_repeat_is_deterministic(gpu):
+    m = gpu.kuhn(16, seed=9)
+    a = run_device(gpu, m.cells, m.n_points, m.scalars)
+    b = run_device(gpu, m.cells, m.n_points, m.scalars, twice=True)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+def test_invalid_cells_rejected(gpu):
+    from tetragpu_amd._native import NativeError
+
+    with pytest.raises(NativeError) as e:
+        gpu.Mesh(np.array([[0, 1, 2, 9]], np.int32), 4, None)
+    assert e.value.status == -4
+    with pytest.raises(NativeError) as e:
+        gpu.Mesh(np.array([[0, 1, 1, 2]], np.int32), 4, None)
+    assert e.value.status == -4
+
+
+def test_approx_max_vv(gpu):
+    m = gpu.kuhn(10, seed=4)
+    with gpu.Mesh(m.cells, m.n_points, m.scalars) as d:
+        assert d.approx_max_vv() == O.approx_max_vv(m.cells, m.n_points) == 96
+
+
+def test_labels6_match_oracle(gpu):
+    m = gpu.kuhn(12, seed=13, scalar="gaussians")
+    with gpu.Mesh(m.cells, m.n_points, m.scalars) as d:
+        d.critpoints()
+        lab = d.labels6()
+        cls = d.classes()
+    assert np.array_equal(lab, O.label6(cls))

@@ -1,0 +1,398 @@
+// C ABI of libtetra_hip.so (include/tetra_c.h): the mesh handle, device
+// buffers it owns, and the dispatch of relation builds.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "pipeline.h"
+#include "relations.h"
+#include "tetra_c.h"
+
+namespace tg {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string &msg) { g_err = msg; }
+const char *last_error() { return g_err.c_str(); }
+
+int hip_fail(hipError_t e, const char *what, const char *file, int line) {
+    char buf[512];
+    std::snprintf(buf, sizeof buf, "%s failed: %s (%s:%d)", what, hipGetErrorString(e),
+                  file, line);
+    set_error(buf);
+    return ERR_HIP;
+}
+
+int DevArr::ensure(size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    if (p && cap >= bytes) return OK;
+    if (p) {
+        hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        p = nullptr;
+        char buf[160];
+        std::snprintf(buf, sizeof buf, "hipMalloc of %zu bytes failed: %s", bytes,
+                      hipGetErrorString(e));
+        set_error(buf);
+        return ERR_ALLOC;
+    }
+    cap = bytes;
+    return OK;
+}
+
+void DevArr::release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
+}  // namespace tg
+
+namespace {
+
+__global__ void k_validate(const int4 *__restrict__ tv4, int64_t T, int64_t V,
+                           int32_t *__restrict__ err) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < T;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int4 q = tv4[t];
+        const bool range = q.x >= 0 && q.y >= 0 && q.z >= 0 && q.w >= 0 && q.x < V &&
+                           q.y < V && q.z < V && q.w < V;
+        const bool distinct = q.x != q.y && q.x != q.z && q.x != q.w && q.y != q.z &&
+                              q.y != q.w && q.z != q.w;
+        if (!(range && distinct)) atomicOr(err, 1);
+    }
+}
+
+__global__ void k_label6(const uint32_t *__restrict__ classes, int64_t V,
+                         uint8_t *__restrict__ out) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t l = classes[3 * v], u = classes[3 * v + 1];
+        uint8_t k;
+        if (l == 0 && u >= 1) k = 0;
+        else if (l >= 1 && u == 0) k = 3;
+        else if (l == 1 && u == 1) k = 5;
+        else if (l == 2 && u == 1) k = 1;
+        else if (l == 1 && u == 2) k = 2;
+        else k = 4;
+        out[v] = k;
+    }
+}
+
+__global__ void k_max_i32(const int32_t *__restrict__ a, int64_t n,
+                          int32_t *__restrict__ out) {
+    int32_t m = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        m = max(m, a[i]);
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) m = max(m, __shfl_xor(m, d, 64));
+    if (__lane_id() == 0) atomicMax(out, m);
+}
+
+int fail(int code, const std::string &msg) {
+    tg::set_error(msg);
+    return code;
+}
+
+}  // namespace
+
+using tg::DevArr;
+
+extern "C" {
+
+const char *tg_last_error(void) { return tg::last_error(); }
+
+int tg_device_count(int *count) {
+    if (!count) return fail(TG_ERR_ARG, "count is NULL");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *count = 0;
+        return fail(TG_ERR_NO_DEVICE, std::string("no HIP device: ") + hipGetErrorString(e));
+    }
+    *count = n;
+    return TG_OK;
+}
+
+int tg_init(int device) {
+    int n = 0;
+    int rc = tg_device_count(&n);
+    if (rc) return rc;
+    if (n == 0) return fail(TG_ERR_NO_DEVICE, "no HIP device visible");
+    if (device < 0 || device >= n) return fail(TG_ERR_ARG, "device index out of range");
+    TG_HIP(hipSetDevice(device));
+    TG_HIP(hipFree(nullptr));
+    return TG_OK;
+}
+
+int tg_mesh_create(tg_mesh **out, const int32_t *cells, int64_t n_cells,
+                   int64_t n_points, const void *scalars, int scalar_f64,
+                   int inputs_on_device, void *stream) {
+    if (!out) return fail(TG_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (n_cells < 0 || n_points < 0 || (n_cells > 0 && !cells))
+        return fail(TG_ERR_ARG, "bad mesh sizes or NULL cells");
+    if (4 * n_cells >= (int64_t)INT32_MAX / 3 || n_points >= (int64_t)INT32_MAX)
+        return fail(TG_ERR_ARG, "mesh too large for 32-bit ids (needs 12*cells < 2^31)");
+    int ndev = 0;
+    int rc = tg_device_count(&ndev);
+    if (rc) return rc;
+    if (ndev == 0) return fail(TG_ERR_NO_DEVICE, "no HIP device visible");
+    auto *m = new tg_mesh();
+    TG_HIP(hipGetDevice(&m->device));
+    m->n_cells = n_cells;
+    m->n_points = n_points;
+    if (stream) {
+        m->stream = (hipStream_t)stream;
+        m->own_stream = false;
+    } else {
+        hipError_t e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete m;
+            return tg::hip_fail(e, "hipStreamCreate", __FILE__, __LINE__);
+        }
+        m->own_stream = true;
+    }
+    auto bail = [&](int code) {
+        tg_mesh_destroy(m);
+        return code;
+    };
+    if ((rc = m->tv.ensure(sizeof(int32_t) * 4 * (size_t)n_cells))) return bail(rc);
+    if ((rc = m->counters.ensure(sizeof(int32_t) * tg::N_COUNTERS))) return bail(rc);
+    if (hipHostMalloc(&m->h_pinned, 64 * sizeof(int64_t)) != hipSuccess)
+        return bail(fail(TG_ERR_ALLOC, "hipHostMalloc failed"));
+    if (n_cells > 0) {
+        hipError_t e = hipMemcpyAsync(m->tv.p, cells, sizeof(int32_t) * 4 * (size_t)n_cells,
+                                      inputs_on_device ? hipMemcpyDeviceToDevice
+                                                       : hipMemcpyHostToDevice,
+                                      m->stream);
+        if (e != hipSuccess) return bail(tg::hip_fail(e, "hipMemcpyAsync(cells)", __FILE__, __LINE__));
+        // validate connectivity once
+        int32_t *err = m->counters.as<int32_t>();
+        if ((rc = tg::fill_i32(err, 0, 1, m->stream))) return bail(rc);
+        unsigned g = std::min<unsigned>(tg::grid_for(n_cells, 256), 16384u);
+        hipLaunchKernelGGL(k_validate, dim3(g), dim3(256), 0, m->stream,
+                           (const int4 *)m->tv.p, n_cells, n_points, err);
+        if (hipGetLastError() != hipSuccess) return bail(fail(TG_ERR_HIP, "k_validate launch"));
+        int32_t herr = 0;
+        if (hipMemcpyAsync(&herr, err, sizeof herr, hipMemcpyDeviceToHost, m->stream) != hipSuccess ||
+            hipStreamSynchronize(m->stream) != hipSuccess)
+            return bail(fail(TG_ERR_HIP, "validation readback failed"));
+        if (herr) return bail(fail(TG_ERR_MESH, "cells hold a vertex id outside [0, n_points) "
+                                                 "or a vertex repeated within a cell"));
+    }
+    if (scalars) {
+        if ((rc = tg_mesh_set_scalars(m, scalars, scalar_f64, inputs_on_device))) return bail(rc);
+    }
+    *out = m;
+    return TG_OK;
+}
+
+int tg_mesh_destroy(tg_mesh *m) {
+    if (!m) return TG_OK;
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    m->release_all();
+    if (m->h_pinned) hipHostFree(m->h_pinned);
+    if (m->own_stream && m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+    return TG_OK;
+}
+
+int tg_mesh_set_scalars(tg_mesh *m, const void *scalars, int scalar_f64, int on_device) {
+    if (!m || !scalars) return fail(TG_ERR_ARG, "NULL mesh or scalars");
+    const size_t bytes = (scalar_f64 ? 8 : 4) * (size_t)m->n_points;
+    int rc = m->scal.ensure(bytes);
+    if (rc) return rc;
+    if (bytes)
+        TG_HIP(hipMemcpyAsync(m->scal.p, scalars, bytes,
+                              on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                              m->stream));
+    m->scal_f64 = scalar_f64 != 0;
+    m->has_scalars = true;
+    m->cp_built = false;
+    return TG_OK;
+}
+
+int tg_mesh_stream(tg_mesh *m, void **stream) {
+    if (!m || !stream) return fail(TG_ERR_ARG, "NULL argument");
+    *stream = (void *)m->stream;
+    return TG_OK;
+}
+
+int tg_mesh_set_config(tg_mesh *m, int32_t lane_cap, int32_t lane_block) {
+    if (!m) return fail(TG_ERR_ARG, "NULL mesh");
+    if (lane_cap != 32 && lane_cap != 64) return fail(TG_ERR_ARG, "lane_cap must be 32 or 64");
+    if (lane_block != 64 && lane_block != 128 && lane_block != 256)
+        return fail(TG_ERR_ARG, "lane_block must be 64, 128 or 256");
+    if (lane_cap == 64 && lane_block == 256)
+        return fail(TG_ERR_ARG, "lane_cap 64 supports lane_block 64 or 128");
+    m->cfg.lane_cap = lane_cap;
+    m->cfg.lane_block = lane_block;
+    return TG_OK;
+}
+
+int tg_build(tg_mesh *m, uint32_t flags) {
+    if (!m) return fail(TG_ERR_ARG, "NULL mesh");
+    return tg::build_relations(*m, flags);
+}
+
+int tg_critpoints(tg_mesh *m) {
+    if (!m) return fail(TG_ERR_ARG, "NULL mesh");
+    if (!m->has_scalars) return fail(TG_ERR_STATE, "mesh has no scalar field");
+    if (m->cp_built) return TG_OK;
+    return tg::run_hot_path(*m);
+}
+
+int tg_vv_critpoints(tg_mesh *m) {
+    if (!m) return fail(TG_ERR_ARG, "NULL mesh");
+    if (!m->has_scalars) return fail(TG_ERR_STATE, "mesh has no scalar field");
+    return tg::run_hot_path(*m);
+}
+
+int tg_sync(tg_mesh *m) {
+    if (!m) return fail(TG_ERR_ARG, "NULL mesh");
+    return tg::sync_and_check(*m);
+}
+
+int tg_mesh_counts(tg_mesh *m, int64_t *n_points, int64_t *n_cells, int64_t *n_edges,
+                   int64_t *n_faces) {
+    if (!m) return fail(TG_ERR_ARG, "NULL mesh");
+    int rc = tg::sync_and_check(*m);
+    if (rc) return rc;
+    if (n_points) *n_points = m->n_points;
+    if (n_cells) *n_cells = m->n_cells;
+    if (n_edges) *n_edges = m->edges_built ? m->n_edges : -1;
+    if (n_faces) *n_faces = m->faces_built ? m->n_faces : -1;
+    return TG_OK;
+}
+
+int tg_relation_shape(tg_mesh *m, int32_t rel, int64_t *rows, int64_t *nnz,
+                      int32_t *width, int32_t *fixed) {
+    if (!m) return fail(TG_ERR_ARG, "NULL mesh");
+    tg::RelView r;
+    int rc = tg::relation_view(*m, rel, r);
+    if (rc) return rc;
+    if (rows) *rows = r.rows;
+    if (nnz) *nnz = r.nnz;
+    if (width) *width = r.width;
+    if (fixed) *fixed = r.offsets ? 0 : 1;
+    return TG_OK;
+}
+
+int tg_relation_copy(tg_mesh *m, int32_t rel, int64_t *offsets, int32_t *values) {
+    if (!m) return fail(TG_ERR_ARG, "NULL mesh");
+    tg::RelView r;
+    int rc = tg::relation_view(*m, rel, r);
+    if (rc) return rc;
+    if (r.nnz * r.width > 0) {
+        if (!values) return fail(TG_ERR_ARG, "values is NULL");
+        TG_HIP(hipMemcpyAsync(values, r.values, sizeof(int32_t) * (size_t)(r.nnz * r.width),
+                              hipMemcpyDeviceToHost, m->stream));
+    }
+    if (r.offsets && offsets) {
+        if (r.offset_bytes == 8) {
+            TG_HIP(hipMemcpyAsync(offsets, r.offsets, sizeof(int64_t) * (size_t)(r.rows + 1),
+                                  hipMemcpyDeviceToHost, m->stream));
+            TG_HIP(hipStreamSynchronize(m->stream));
+        } else {
+            int32_t *tmp = (int32_t *)std::malloc(sizeof(int32_t) * (size_t)(r.rows + 1));
+            if (!tmp) return fail(TG_ERR_ALLOC, "host allocation failed");
+            hipError_t e = hipMemcpyAsync(tmp, r.offsets, sizeof(int32_t) * (size_t)(r.rows + 1),
+                                          hipMemcpyDeviceToHost, m->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+            if (e != hipSuccess) {
+                std::free(tmp);
+                return tg::hip_fail(e, "offset copy", __FILE__, __LINE__);
+            }
+            for (int64_t i = 0; i <= r.rows; i++) offsets[i] = tmp[i];
+            std::free(tmp);
+        }
+    }
+    TG_HIP(hipStreamSynchronize(m->stream));
+    return TG_OK;
+}
+
+int tg_relation_device(tg_mesh *m, int32_t rel, const void **d_offsets, int32_t *offset_bytes,
+                       const int32_t **d_values) {
+    if (!m) return fail(TG_ERR_ARG, "NULL mesh");
+    tg::RelView r;
+    int rc = tg::relation_view(*m, rel, r);
+    if (rc) return rc;
+    if (d_offsets) *d_offsets = r.offsets;
+    if (offset_bytes) *offset_bytes = r.offsets ? r.offset_bytes : 0;
+    if (d_values) *d_values = r.values;
+    return TG_OK;
+}
+
+int tg_classes_device(tg_mesh *m, const uint32_t **d_classes3) {
+    if (!m || !d_classes3) return fail(TG_ERR_ARG, "NULL argument");
+    if (!m->cp_built) return fail(TG_ERR_STATE, "critical points not computed");
+    *d_classes3 = (const uint32_t *)m->classes.p;
+    return TG_OK;
+}
+
+int tg_classes_copy(tg_mesh *m, uint32_t *classes3) {
+    if (!m || (!classes3 && m->n_points > 0)) return fail(TG_ERR_ARG, "NULL argument");
+    if (!m->cp_built) return fail(TG_ERR_STATE, "critical points not computed");
+    int rc = tg::sync_and_check(*m);
+    if (rc) return rc;
+    if (m->n_points > 0)
+        TG_HIP(hipMemcpyAsync(classes3, m->classes.p, sizeof(uint32_t) * 3 * (size_t)m->n_points,
+                              hipMemcpyDeviceToHost, m->stream));
+    TG_HIP(hipStreamSynchronize(m->stream));
+    return TG_OK;
+}
+
+int tg_labels6_copy(tg_mesh *m, uint8_t *labels) {
+    if (!m || (!labels && m->n_points > 0)) return fail(TG_ERR_ARG, "NULL argument");
+    if (!m->cp_built) return fail(TG_ERR_STATE, "critical points not computed");
+    if (m->n_points == 0) return TG_OK;
+    int rc = m->labels.ensure((size_t)m->n_points);
+    if (rc) return rc;
+    unsigned g = std::min<unsigned>(tg::grid_for(m->n_points, 256), 16384u);
+    hipLaunchKernelGGL(k_label6, dim3(g), dim3(256), 0, m->stream,
+                       (const uint32_t *)m->classes.p, m->n_points, (uint8_t *)m->labels.p);
+    TG_LAUNCH_CHECK("k_label6");
+    rc = tg::sync_and_check(*m);
+    if (rc) return rc;
+    TG_HIP(hipMemcpyAsync(labels, m->labels.p, (size_t)m->n_points, hipMemcpyDeviceToHost,
+                          m->stream));
+    TG_HIP(hipStreamSynchronize(m->stream));
+    return TG_OK;
+}
+
+int tg_approx_max_vv(tg_mesh *m, int32_t *max_vv) {
+    if (!m || !max_vv) return fail(TG_ERR_ARG, "NULL argument");
+    int rc = tg::ensure_vt_corners(*m);
+    if (rc) return rc;
+    int32_t *d = m->counters.as<int32_t>() + tg::N_COUNTERS - 1;
+    if ((rc = tg::fill_i32(d, 0, 1, m->stream))) return rc;
+    if (m->n_points > 0) {
+        unsigned g = std::min<unsigned>(tg::grid_for(m->n_points, 256), 4096u);
+        hipLaunchKernelGGL(k_max_i32, dim3(g), dim3(256), 0, m->stream,
+                           (const int32_t *)m->vt_cnt.p, m->n_points, d);
+        TG_LAUNCH_CHECK("k_max_i32");
+    }
+    int32_t mx = 0;
+    TG_HIP(hipMemcpyAsync(&mx, d, sizeof mx, hipMemcpyDeviceToHost, m->stream));
+    TG_HIP(hipStreamSynchronize(m->stream));
+    mx *= 3;
+    *max_vv = ((mx + 31) / 32) * 32;
+    return TG_OK;
+}
+
+int tg_device_free(void *p) {
+    if (p) TG_HIP(hipFree(p));
+    return TG_OK;
+}
+
+}  // extern "C"

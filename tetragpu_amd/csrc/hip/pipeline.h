@@ -1,0 +1,64 @@
+// Host launchers of the relation and critical-point pipelines.  Every
+// launcher only enqueues work on the given stream (no host synchronisation,
+// no allocation, no hipMemset), so a launch sequence can be captured into a
+// hipGraph and replayed.
+#pragma once
+
+#include "common.h"
+
+namespace tg {
+
+// Device counters shared by one pipeline run.
+enum Counter : int {
+    C_HEAVY = 0,   // vertices routed to the serial VV kernel
+    C_FIX = 1,     // vertices needing the row-membership link pass (deg<=64)
+    C_FIXBIG = 2,  // same, deg > 64
+    C_ERR = 3,     // error bits (ERRBIT_*)
+    C_MAXDEG = 4,  // largest VV degree seen
+    N_COUNTERS = 8
+};
+enum ErrBit : int { ERRBIT_CAPACITY = 1 };
+
+// Largest vertex degree the serial per-vertex kernels hold in LDS.
+constexpr int HEAVY_CAP = 4096;
+
+// Tunables of the VV + critical point pipeline (picked by the autotuner).
+struct VvCpConfig {
+    int lane_cap = 32;     // neighbours held per lane (32 or 64)
+    int lane_block = 128;  // lanes (vertices) per workgroup
+    int corner_block = 256;
+};
+
+// Device buffers of one VV + critical point run.  Sizes:
+//   vt_cnt, vt_cur, deg : V      vt_off, vv_off : V+1
+//   vtc : 4T                     scratch : 12T (padded rows at 3*vt_off[v])
+//   vv_cols : 12T (capacity)     classes : 3V
+//   heavy, fix, fixbig : V       counters : N_COUNTERS
+//   scan_tmp : scan_tmp_bytes(V)
+struct VvCpBuffers {
+    const int32_t *tv = nullptr;
+    const void *scalars = nullptr;
+    bool scalars_f64 = false;
+    int64_t n_cells = 0, n_points = 0;
+    int32_t *vt_cnt = nullptr, *vt_off = nullptr, *vt_cur = nullptr,
+            *vtc = nullptr, *scratch = nullptr, *deg = nullptr,
+            *vv_off = nullptr, *vv_cols = nullptr;
+    uint32_t *classes = nullptr;
+    int32_t *heavy = nullptr, *fix = nullptr, *fixbig = nullptr,
+            *counters = nullptr;
+    void *scan_tmp = nullptr;
+};
+
+// vertex -> corner incidence (p = 4t + j) grouped by vertex, unordered rows.
+int launch_vt_corners(VvCpBuffers &b, int block, hipStream_t s);
+// Sort every vt row ascending (needed by the id-assigning builders).
+int launch_sort_rows_i32(const int32_t *off, int32_t *vals, int64_t rows,
+                         hipStream_t s);
+// VV CSR (ascending rows) + classes[3V]; assumes launch_vt_corners ran.
+int launch_vv_cp(VvCpBuffers &b, const VvCpConfig &cfg, bool classify,
+                 hipStream_t s);
+// Whole pipeline: counters reset, VT corners, VV, critical points.
+int run_vv_cp(VvCpBuffers &b, const VvCpConfig &cfg, bool classify,
+              hipStream_t s);
+
+}  // namespace tg

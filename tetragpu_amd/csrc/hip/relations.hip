@@ -1,0 +1,228 @@
+// Relation dispatch behind tg_build / tg_relation_*: buffer management of the
+// mesh handle, the VV + critical point hot path, VT, and (ids.hip,
+// derived.hip) the id-assigning and derived relations.
+#include <algorithm>
+#include <cstdio>
+
+#include "ids.h"
+#include "relations.h"
+#include "tetra_c.h"
+
+namespace {
+
+__global__ void k_corner_to_tet(const int32_t *__restrict__ vtc, int64_t n,
+                                int32_t *__restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = vtc[i] >> 2;
+}
+
+int fail(int code, const char *msg) {
+    tg::set_error(msg);
+    return code;
+}
+
+// Read one int32 from device memory (synchronises the stream).
+int read_i32(tg_mesh &m, const void *src, int64_t index, int32_t &out) {
+    int32_t *h = (int32_t *)m.h_pinned;
+    TG_HIP(hipMemcpyAsync(h, (const int32_t *)src + index, sizeof(int32_t),
+                          hipMemcpyDeviceToHost, m.stream));
+    TG_HIP(hipStreamSynchronize(m.stream));
+    out = *h;
+    return tg::OK;
+}
+
+}  // namespace
+
+void tg_mesh::release_all() {
+    tg::DevArr *all[] = {&tv,       &scal,     &vt_cnt,   &vt_off,   &vt_cur,   &vtc,
+                         &scratch,  &deg,      &vv_off,   &vv_cols,  &classes,  &heavy,
+                         &fix,      &fixbig,   &counters, &scan_tmp, &labels,   &vt_vals,
+                         &edge_first, &ve_off, &ve_vals,  &ev,       &te,       &vv_eid,
+                         &et_off,   &et_vals,  &face_off, &vf_vals,  &fv,       &tf,
+                         &fe,       &ft,       &ef_off,   &ef_vals,  &tt_off,   &tt_vals,
+                         &ff_off,   &ff_vals,  &ee_off,   &ee_vals,  &bitmap,   &rank_tmp,
+                         &tmp_a,    &tmp_b,    &tmp_c};
+    for (auto *a : all) a->release();
+}
+
+namespace tg {
+
+VvCpBuffers hot_buffers(tg_mesh &m) {
+    VvCpBuffers b;
+    b.tv = m.tv.as<int32_t>();
+    b.scalars = m.scal.p;
+    b.scalars_f64 = m.scal_f64;
+    b.n_cells = m.n_cells;
+    b.n_points = m.n_points;
+    b.vt_cnt = m.vt_cnt.as<int32_t>();
+    b.vt_off = m.vt_off.as<int32_t>();
+    b.vt_cur = m.vt_cur.as<int32_t>();
+    b.vtc = m.vtc.as<int32_t>();
+    b.scratch = m.scratch.as<int32_t>();
+    b.deg = m.deg.as<int32_t>();
+    b.vv_off = m.vv_off.as<int32_t>();
+    b.vv_cols = m.vv_cols.as<int32_t>();
+    b.classes = m.classes.as<uint32_t>();
+    b.heavy = m.heavy.as<int32_t>();
+    b.fix = m.fix.as<int32_t>();
+    b.fixbig = m.fixbig.as<int32_t>();
+    b.counters = m.counters.as<int32_t>();
+    b.scan_tmp = m.scan_tmp.p;
+    return b;
+}
+
+int prepare_hot_path(tg_mesh &m) {
+    if (m.hot_ready) return OK;
+    const size_t V = (size_t)m.n_points, T = (size_t)m.n_cells, I = sizeof(int32_t);
+    int rc;
+    if ((rc = m.vt_cnt.ensure(I * V)) || (rc = m.vt_off.ensure(I * (V + 1))) ||
+        (rc = m.vt_cur.ensure(I * V)) || (rc = m.vtc.ensure(I * 4 * T)) ||
+        (rc = m.scratch.ensure(I * 12 * T)) || (rc = m.deg.ensure(I * V)) ||
+        (rc = m.vv_off.ensure(I * (V + 1))) || (rc = m.vv_cols.ensure(I * 12 * T)) ||
+        (rc = m.classes.ensure(I * 3 * V)) || (rc = m.heavy.ensure(I * V)) ||
+        (rc = m.fix.ensure(I * V)) || (rc = m.fixbig.ensure(I * V)) ||
+        (rc = m.counters.ensure(I * N_COUNTERS)) ||
+        (rc = m.scan_tmp.ensure(scan_tmp_bytes(std::max<int64_t>(m.n_points, 12 * m.n_cells)))))
+        return rc;
+    m.hot_ready = true;
+    return OK;
+}
+
+int run_hot_path(tg_mesh &m) {
+    int rc = prepare_hot_path(m);
+    if (rc) return rc;
+    VvCpBuffers b = hot_buffers(m);
+    rc = run_vv_cp(b, m.cfg, true, m.stream);
+    if (rc) return rc;
+    m.vt_corners_built = true;
+    m.vt_sorted = false;
+    m.vv_built = true;
+    m.cp_built = true;
+    m.counters_pending = true;
+    return OK;
+}
+
+int ensure_vt_corners(tg_mesh &m) {
+    if (m.vt_corners_built) return OK;
+    int rc = prepare_hot_path(m);
+    if (rc) return rc;
+    VvCpBuffers b = hot_buffers(m);
+    if ((rc = launch_vt_corners(b, m.cfg.corner_block, m.stream))) return rc;
+    m.vt_corners_built = true;
+    m.vt_sorted = false;
+    return OK;
+}
+
+int ensure_vt_sorted(tg_mesh &m) {
+    int rc = ensure_vt_corners(m);
+    if (rc) return rc;
+    if (m.vt_sorted) return OK;
+    if ((rc = launch_sort_rows_i32(m.vt_off.as<int32_t>(), m.vtc.as<int32_t>(), m.n_points,
+                                   m.stream)))
+        return rc;
+    m.vt_sorted = true;
+    return OK;
+}
+
+int ensure_vv(tg_mesh &m) {
+    if (m.vv_built) return OK;
+    int rc = prepare_hot_path(m);
+    if (rc) return rc;
+    if ((rc = fill_i32(m.counters.as<int32_t>(), 0, N_COUNTERS, m.stream))) return rc;
+    if ((rc = ensure_vt_corners(m))) return rc;
+    VvCpBuffers b = hot_buffers(m);
+    if ((rc = launch_vv_cp(b, m.cfg, false, m.stream))) return rc;
+    m.vv_built = true;
+    m.counters_pending = true;
+    return OK;
+}
+
+static int ensure_vt(tg_mesh &m) {
+    if (m.vt_built) return OK;
+    int rc = ensure_vt_sorted(m);
+    if (rc) return rc;
+    const int64_t n = 4 * m.n_cells;
+    if ((rc = m.vt_vals.ensure(sizeof(int32_t) * (size_t)n))) return rc;
+    if (n > 0) {
+        unsigned g = std::min<unsigned>(grid_for(n, 256), 65536u);
+        hipLaunchKernelGGL(k_corner_to_tet, dim3(g), dim3(256), 0, m.stream,
+                           m.vtc.as<int32_t>(), n, m.vt_vals.as<int32_t>());
+        TG_LAUNCH_CHECK("k_corner_to_tet");
+    }
+    m.vt_built = true;
+    return OK;
+}
+
+int build_relations(tg_mesh &m, uint32_t flags) {
+    int rc = OK;
+    auto want = [&](int rel) { return (flags >> rel) & 1u; };
+    if (want(TG_REL_VV) && (rc = ensure_vv(m))) return rc;
+    if (want(TG_REL_VT) && (rc = ensure_vt(m))) return rc;
+    // id-assigning relations and those derived from them
+    const bool need_edges = want(TG_REL_TE) || want(TG_REL_EV) || want(TG_REL_ET) ||
+                            want(TG_REL_FE) || want(TG_REL_EF) || want(TG_REL_EE) ||
+                            want(TG_REL_VE);
+    const bool need_faces = want(TG_REL_TF) || want(TG_REL_FV) || want(TG_REL_FE) ||
+                            want(TG_REL_FT) || want(TG_REL_EF) || want(TG_REL_FF) ||
+                            want(TG_REL_VF);
+    if (need_edges && (rc = ensure_edges(m))) return rc;
+    if (need_faces && (rc = ensure_faces(m))) return rc;
+    if (want(TG_REL_ET) && (rc = ensure_et(m))) return rc;
+    if ((want(TG_REL_FE) || want(TG_REL_EF)) && (rc = ensure_fe(m))) return rc;
+    if ((want(TG_REL_FT) || want(TG_REL_FF)) && (rc = ensure_ft(m))) return rc;
+    if (want(TG_REL_EF) && (rc = ensure_ef(m))) return rc;
+    if (want(TG_REL_TT) && (rc = ensure_tt(m))) return rc;
+    if (want(TG_REL_FF) && (rc = ensure_ff(m))) return rc;
+    if (want(TG_REL_EE) && (rc = ensure_ee(m))) return rc;
+    return OK;
+}
+
+int sync_and_check(tg_mesh &m) {
+    TG_HIP(hipStreamSynchronize(m.stream));
+    if (!m.counters_pending) return OK;
+    m.counters_pending = false;
+    int32_t err = 0;
+    int rc = read_i32(m, m.counters.p, C_ERR, err);
+    if (rc) return rc;
+    if (err & ERRBIT_CAPACITY) {
+        char buf[160];
+        std::snprintf(buf, sizeof buf,
+                      "a vertex has more than %d distinct neighbours; its row and class "
+                      "were not computed",
+                      HEAVY_CAP);
+        set_error(buf);
+        return ERR_CAPACITY;
+    }
+    return OK;
+}
+
+int relation_view(tg_mesh &m, int32_t rel, RelView &r) {
+    int rc = sync_and_check(m);
+    if (rc) return rc;
+    r = RelView();
+    int32_t nnz = 0;
+    switch (rel) {
+    case TG_REL_VV:
+        if (!m.vv_built) return fail(ERR_STATE, "VV not built");
+        if ((rc = read_i32(m, m.vv_off.p, m.n_points, nnz))) return rc;
+        r.rows = m.n_points;
+        r.nnz = nnz;
+        r.offsets = m.vv_off.p;
+        r.offset_bytes = 4;
+        r.values = m.vv_cols.as<int32_t>();
+        return OK;
+    case TG_REL_VT:
+        if (!m.vt_built) return fail(ERR_STATE, "VT not built");
+        r.rows = m.n_points;
+        r.nnz = 4 * m.n_cells;
+        r.offsets = m.vt_off.p;
+        r.offset_bytes = 4;
+        r.values = m.vt_vals.as<int32_t>();
+        return OK;
+    default:
+        return id_relation_view(m, rel, r);
+    }
+}
+
+}  // namespace tg

@@ -1,0 +1,581 @@
+// VV relation (vertex -> distinct co-vertices, ascending rows) and critical-
+// point classification on gfx950.
+//
+// Reference: the padded, atomically appended VV_kernel
+// (src/cuda_extraction.cu:960-1057) and the block-per-vertex critPoints
+// kernel (src/alg/critPoints.cu:49-338).  Here:
+//   1. vertex -> corner incidence by counting (atomics on V counters), an
+//      exclusive scan and an atomic-cursor scatter;
+//   2. one lane per vertex walks its tets, keeps its distinct neighbours in
+//      an LDS list laid out lane-interleaved (element i of lane l at
+//      i*BLOCK + l, so concurrent lanes never share a bank), and unions
+//      same-class neighbours that share one of its tets with a union-find in
+//      LDS;  vertices with more neighbours than the lane list holds go to a
+//      serial per-workgroup kernel with a 4096-entry list;
+//   3. the sorted row goes to a padded scratch row at 3*vt_off[v] (a vertex
+//      in d tets has at most 3d neighbours), rows are compacted after a scan
+//      of the degrees;
+//   4. the shared-tet link graph is a subgraph of the reference's link graph
+//      (link edge (a,c) iff c is in VV(a), critPoints.cu:206-246), so a class
+//      whose shared-tet components number 0 or 1 already has the reference
+//      count; only vertices with >= 2 components in a class are recomputed
+//      with the row-membership rule on the finished VV rows.
+#include "pipeline.h"
+
+#include <algorithm>
+
+namespace {
+
+using tg::C_ERR;
+using tg::C_FIX;
+using tg::C_FIXBIG;
+using tg::C_HEAVY;
+using tg::C_MAXDEG;
+using tg::HEAVY_CAP;
+
+// critPoints.cu:136-140: equal scalars fall back to the id comparison.
+template <typename S>
+__device__ __forceinline__ bool is_lower(S sa, S sv, int32_t a, int32_t v) {
+    return (sa == sv) ? (a < v) : (sa < sv);
+}
+
+// Numeric verdict written by critPoints.cu:303-330 given the lower and upper
+// component counts: 1 minimum, 2 maximum, 3 regular, 4 saddle.
+__device__ __forceinline__ uint32_t verdict(uint32_t nl, uint32_t nu) {
+    if (nl >= 1 && nu == 0) return 2u;
+    if (nl == 0 && nu >= 1) return 1u;
+    if (nl == 1 && nu == 1) return 3u;
+    return 4u;
+}
+
+// Append v to a device list with one atomic per wave.
+__device__ __forceinline__ void wave_append(int32_t *list, int32_t *counter,
+                                            int32_t v, bool pred) {
+    const uint64_t mask = __ballot(pred);
+    if (!mask) return;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    int32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (int32_t)__popcll(mask));
+    base = __shfl(base, leader, 64);
+    if (pred) {
+        const uint64_t below = mask & ((1ull << lane) - 1ull);
+        list[base + __popcll(below)] = v;
+    }
+}
+
+// ---- vertex -> corner incidence -------------------------------------------
+__global__ void k_count_corners(const int4 *__restrict__ tv4, int64_t T,
+                                int32_t *__restrict__ cnt) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < T;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int4 q = tv4[t];
+        atomicAdd(&cnt[q.x], 1);
+        atomicAdd(&cnt[q.y], 1);
+        atomicAdd(&cnt[q.z], 1);
+        atomicAdd(&cnt[q.w], 1);
+    }
+}
+
+__global__ void k_fill_corners(const int4 *__restrict__ tv4, int64_t T,
+                               const int32_t *__restrict__ off,
+                               int32_t *__restrict__ cur,
+                               int32_t *__restrict__ vtc) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < T;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int4 q = tv4[t];
+        const int32_t p = (int32_t)(4 * t);
+        vtc[off[q.x] + atomicAdd(&cur[q.x], 1)] = p;
+        vtc[off[q.y] + atomicAdd(&cur[q.y], 1)] = p + 1;
+        vtc[off[q.z] + atomicAdd(&cur[q.z], 1)] = p + 2;
+        vtc[off[q.w] + atomicAdd(&cur[q.w], 1)] = p + 3;
+    }
+}
+
+// Lane-serial sort of one row: insertion sort for short rows, heapsort
+// otherwise.
+__device__ void sort_row(int32_t *a, int32_t n) {
+    if (n <= 32) {
+        for (int32_t i = 1; i < n; i++) {
+            const int32_t x = a[i];
+            int32_t j = i - 1;
+            while (j >= 0 && a[j] > x) {
+                a[j + 1] = a[j];
+                j--;
+            }
+            a[j + 1] = x;
+        }
+        return;
+    }
+    auto sift = [&](int32_t root, int32_t end) {
+        while (2 * root + 1 < end) {
+            int32_t child = 2 * root + 1;
+            if (child + 1 < end && a[child] < a[child + 1]) child++;
+            if (a[root] >= a[child]) return;
+            const int32_t t = a[root];
+            a[root] = a[child];
+            a[child] = t;
+            root = child;
+        }
+    };
+    for (int32_t i = n / 2 - 1; i >= 0; i--) sift(i, n);
+    for (int32_t end = n - 1; end > 0; end--) {
+        const int32_t t = a[0];
+        a[0] = a[end];
+        a[end] = t;
+        sift(0, end);
+    }
+}
+
+__global__ void k_sort_rows(const int32_t *__restrict__ off, int32_t *vals,
+                            int64_t rows) {
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < rows;
+         r += (int64_t)gridDim.x * blockDim.x)
+        sort_row(vals + off[r], off[r + 1] - off[r]);
+}
+
+// ---- lane-per-vertex VV + shared-tet link components ----------------------
+template <int CAP, int BLOCK, typename S>
+__global__ __launch_bounds__(BLOCK) void k_vv_cp_lane(
+    const int4 *__restrict__ tv4, const int32_t *__restrict__ vt_off,
+    const int32_t *__restrict__ vtc, const S *__restrict__ scal, int32_t V,
+    int32_t *__restrict__ scratch, int32_t *__restrict__ deg,
+    uint32_t *__restrict__ classes, int32_t *__restrict__ heavy,
+    int32_t *__restrict__ fix, int32_t *__restrict__ counters, int classify) {
+    static_assert(CAP <= 64, "lane list is indexed by a 64-bit class mask");
+    __shared__ int32_t s_list[CAP * BLOCK];
+    __shared__ uint8_t s_par[CAP * BLOCK];
+    const int tid = threadIdx.x;
+    const int32_t v = (int32_t)blockIdx.x * BLOCK + tid;
+    const bool active = v < V;
+    int32_t *list = s_list + tid;
+    uint8_t *par = s_par + tid;
+    int m = 0;
+    bool over = false;
+    uint64_t low = 0;
+    uint32_t nl = 0, nu = 0;
+    int32_t beg = 0;
+    if (active) {
+        beg = vt_off[v];
+        const int32_t end = vt_off[v + 1];
+        const S sv = classify ? scal[v] : S(0);
+        auto find_or_insert = [&](int32_t u) -> int {
+            for (int i = 0; i < m; i++)
+                if (list[i * BLOCK] == u) return i;
+            if (m == CAP) return -1;
+            list[m * BLOCK] = u;
+            par[m * BLOCK] = (uint8_t)m;
+            if (classify && is_lower(scal[u], sv, u, v)) low |= 1ull << m;
+            return m++;
+        };
+        auto root = [&](int x) {
+            int p;
+            while ((p = par[x * BLOCK]) != x) {
+                const int g = par[p * BLOCK];
+                par[x * BLOCK] = (uint8_t)g;
+                x = g;
+            }
+            return x;
+        };
+        auto unite = [&](int a, int b) {
+            if (((low >> a) ^ (low >> b)) & 1ull) return;
+            a = root(a);
+            b = root(b);
+            if (a == b) return;
+            if (a < b) par[b * BLOCK] = (uint8_t)a;
+            else par[a * BLOCK] = (uint8_t)b;
+        };
+        for (int32_t k = beg; k < end; k++) {
+            const int32_t p = vtc[k];
+            const int4 q = tv4[p >> 2];
+            const int jv = p & 3;
+            const int32_t o0 = jv == 0 ? q.y : q.x;
+            const int32_t o1 = jv <= 1 ? q.z : q.y;
+            const int32_t o2 = jv <= 2 ? q.w : q.z;
+            const int i0 = find_or_insert(o0);
+            const int i1 = find_or_insert(o1);
+            const int i2 = find_or_insert(o2);
+            if ((i0 | i1 | i2) < 0) {
+                over = true;
+                break;
+            }
+            if (classify) {
+                unite(i0, i1);
+                unite(i0, i2);
+                unite(i1, i2);
+            }
+        }
+        if (!over) {
+            if (classify) {
+                for (int i = 0; i < m; i++)
+                    if (par[i * BLOCK] == i) {
+                        if ((low >> i) & 1ull) nl++;
+                        else nu++;
+                    }
+                classes[3 * (int64_t)v] = nl;
+                classes[3 * (int64_t)v + 1] = nu;
+                classes[3 * (int64_t)v + 2] = verdict(nl, nu);
+            }
+            deg[v] = m;
+            int32_t *dst = scratch + 3 * (int64_t)beg;
+            for (int i = 0; i < m; i++) {
+                const int32_t x = list[i * BLOCK];
+                int r = 0;
+                for (int j = 0; j < m; j++) r += list[j * BLOCK] < x;
+                dst[r] = x;
+            }
+        }
+    }
+    wave_append(heavy, counters + C_HEAVY, v, active && over);
+    wave_append(fix, counters + C_FIX, v,
+                active && !over && classify && (nl >= 2 || nu >= 2));
+    int wmax = (active && !over) ? m : 0;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) wmax = max(wmax, __shfl_xor(wmax, d, 64));
+    if (__lane_id() == 0 && wmax > 0) atomicMax(counters + C_MAXDEG, wmax);
+}
+
+// Serial fallback for vertices whose neighbour list overflows the lane list:
+// lane 0 of each workgroup handles one vertex with a 4096-entry LDS list.
+template <typename S>
+__global__ __launch_bounds__(64) void k_vv_cp_heavy(
+    const int4 *__restrict__ tv4, const int32_t *__restrict__ vt_off,
+    const int32_t *__restrict__ vtc, const S *__restrict__ scal,
+    int32_t *__restrict__ scratch, int32_t *__restrict__ deg,
+    uint32_t *__restrict__ classes, const int32_t *__restrict__ heavy,
+    int32_t *__restrict__ fix, int32_t *__restrict__ fixbig,
+    int32_t *__restrict__ counters, int classify) {
+    __shared__ int32_t list[HEAVY_CAP];
+    __shared__ uint16_t par[HEAVY_CAP];
+    __shared__ uint8_t lowf[HEAVY_CAP];
+    if (threadIdx.x != 0) return;
+    const int32_t n = counters[C_HEAVY];
+    for (int32_t h = blockIdx.x; h < n; h += gridDim.x) {
+        const int32_t v = heavy[h];
+        const int32_t beg = vt_off[v], end = vt_off[v + 1];
+        const S sv = classify ? scal[v] : S(0);
+        int m = 0;
+        bool over = false;
+        auto find_or_insert = [&](int32_t u) -> int {
+            for (int i = 0; i < m; i++)
+                if (list[i] == u) return i;
+            if (m == HEAVY_CAP) return -1;
+            list[m] = u;
+            par[m] = (uint16_t)m;
+            lowf[m] = classify ? (uint8_t)is_lower(scal[u], sv, u, v) : 0;
+            return m++;
+        };
+        auto root = [&](int x) {
+            while (par[x] != x) {
+                par[x] = par[par[x]];
+                x = par[x];
+            }
+            return x;
+        };
+        auto unite = [&](int a, int b) {
+            if (lowf[a] != lowf[b]) return;
+            a = root(a);
+            b = root(b);
+            if (a == b) return;
+            if (a < b) par[b] = (uint16_t)a;
+            else par[a] = (uint16_t)b;
+        };
+        for (int32_t k = beg; k < end && !over; k++) {
+            const int32_t p = vtc[k];
+            const int4 q = tv4[p >> 2];
+            const int jv = p & 3;
+            const int i0 = find_or_insert(jv == 0 ? q.y : q.x);
+            const int i1 = find_or_insert(jv <= 1 ? q.z : q.y);
+            const int i2 = find_or_insert(jv <= 2 ? q.w : q.z);
+            if ((i0 | i1 | i2) < 0) {
+                over = true;
+                break;
+            }
+            if (classify) {
+                unite(i0, i1);
+                unite(i0, i2);
+                unite(i1, i2);
+            }
+        }
+        if (over) {
+            atomicOr(counters + C_ERR, tg::ERRBIT_CAPACITY);
+            deg[v] = 0;
+            if (classify) {
+                classes[3 * (int64_t)v] = 0;
+                classes[3 * (int64_t)v + 1] = 0;
+                classes[3 * (int64_t)v + 2] = 0;
+            }
+            continue;
+        }
+        uint32_t nl = 0, nu = 0;
+        if (classify) {
+            for (int i = 0; i < m; i++)
+                if (par[i] == i) {
+                    if (lowf[i]) nl++;
+                    else nu++;
+                }
+            classes[3 * (int64_t)v] = nl;
+            classes[3 * (int64_t)v + 1] = nu;
+            classes[3 * (int64_t)v + 2] = verdict(nl, nu);
+        }
+        deg[v] = m;
+        atomicMax(counters + C_MAXDEG, m);
+        // ascending row via insertion sort in LDS
+        for (int i = 1; i < m; i++) {
+            const int32_t x = list[i];
+            int j = i - 1;
+            while (j >= 0 && list[j] > x) {
+                list[j + 1] = list[j];
+                j--;
+            }
+            list[j + 1] = x;
+        }
+        int32_t *dst = scratch + 3 * (int64_t)beg;
+        for (int i = 0; i < m; i++) dst[i] = list[i];
+        if (classify && (nl >= 2 || nu >= 2)) {
+            if (m <= 64) fix[atomicAdd(counters + C_FIX, 1)] = v;
+            else fixbig[atomicAdd(counters + C_FIXBIG, 1)] = v;
+        }
+    }
+}
+
+// ---- compaction of padded rows into the CSR ---------------------------------
+__global__ __launch_bounds__(256) void k_vv_compact(
+    const int32_t *__restrict__ vt_off, const int32_t *__restrict__ vv_off,
+    const int32_t *__restrict__ scratch, int32_t V, int32_t *__restrict__ cols) {
+    __shared__ int32_t s_off[257];
+    __shared__ int64_t s_src[256];
+    const int32_t v0 = (int32_t)blockIdx.x * 256;
+    const int nv = min(256, V - v0);
+    for (int i = threadIdx.x; i <= nv; i += 256) {
+        s_off[i] = vv_off[v0 + i];
+        if (i < nv) s_src[i] = 3 * (int64_t)vt_off[v0 + i];
+    }
+    __syncthreads();
+    const int32_t ob = s_off[0], oe = s_off[nv];
+    for (int32_t o = ob + threadIdx.x; o < oe; o += 256) {
+        int lo = 0, hi = nv - 1;  // last row r with s_off[r] <= o
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_off[mid] <= o) lo = mid;
+            else hi = mid - 1;
+        }
+        cols[o] = scratch[s_src[lo] + (o - s_off[lo])];
+    }
+}
+
+// ---- row-membership link components for flagged vertices -------------------
+// Link edge (a, c) between two neighbours of v exists iff c is in VV(a)
+// (critPoints.cu:231-233).  Each undirected edge is found once, from the
+// smaller endpoint, by merging the ascending rows VV(a) (entries > a) and
+// VV(v) (entries after a).
+template <typename S>
+__device__ void link_components(int32_t v, const int32_t *__restrict__ off,
+                                const int32_t *__restrict__ cols,
+                                const S *__restrict__ scal, int d,
+                                const int32_t *nb, uint16_t *par, int pstride,
+                                uint8_t *lowf, int lstride, uint32_t &nl,
+                                uint32_t &nu) {
+    const S sv = scal[v];
+    for (int i = 0; i < d; i++) {
+        const int32_t a = nb[i];
+        par[i * pstride] = (uint16_t)i;
+        lowf[i * lstride] = (uint8_t)is_lower(scal[a], sv, a, v);
+    }
+    auto root = [&](int x) {
+        int p;
+        while ((p = par[x * pstride]) != x) {
+            const int g = par[p * pstride];
+            par[x * pstride] = (uint16_t)g;
+            x = g;
+        }
+        return x;
+    };
+    for (int ia = 0; ia < d - 1; ia++) {
+        const int32_t a = nb[ia];
+        int32_t j = off[a];
+        const int32_t je = off[a + 1];
+        // first entry of VV(a) greater than a
+        int32_t lo = j, hi = je;
+        while (lo < hi) {
+            const int32_t mid = (lo + hi) >> 1;
+            if (cols[mid] <= a) lo = mid + 1;
+            else hi = mid;
+        }
+        j = lo;
+        int ic = ia + 1;
+        const uint8_t la = lowf[ia * lstride];
+        while (j < je && ic < d) {
+            const int32_t c = cols[j], w = nb[ic];
+            if (c < w) j++;
+            else if (w < c) ic++;
+            else {
+                if (lowf[ic * lstride] == la) {
+                    int ra = root(ia), rc = root(ic);
+                    if (ra != rc) {
+                        if (ra < rc) par[rc * pstride] = (uint16_t)ra;
+                        else par[ra * pstride] = (uint16_t)rc;
+                    }
+                }
+                j++;
+                ic++;
+            }
+        }
+    }
+    nl = nu = 0;
+    for (int i = 0; i < d; i++)
+        if (par[i * pstride] == i) {
+            if (lowf[i * lstride]) nl++;
+            else nu++;
+        }
+}
+
+template <int BLOCK, typename S>
+__global__ __launch_bounds__(BLOCK) void k_cp_fix_lane(
+    const int32_t *__restrict__ off, const int32_t *__restrict__ cols,
+    const S *__restrict__ scal, const int32_t *__restrict__ fix,
+    int32_t *__restrict__ fixbig, int32_t *__restrict__ counters,
+    uint32_t *__restrict__ classes) {
+    __shared__ uint16_t s_par[64 * BLOCK];
+    __shared__ uint8_t s_low[64 * BLOCK];
+    const int32_t n = counters[C_FIX];
+    for (int32_t i = (int32_t)blockIdx.x * BLOCK + threadIdx.x; i < n;
+         i += (int32_t)gridDim.x * BLOCK) {
+        const int32_t v = fix[i];
+        const int32_t b = off[v];
+        const int d = off[v + 1] - b;
+        if (d > 64) {
+            fixbig[atomicAdd(counters + C_FIXBIG, 1)] = v;
+            continue;
+        }
+        uint32_t nl, nu;
+        link_components(v, off, cols, scal, d, cols + b, s_par + threadIdx.x, BLOCK,
+                        s_low + threadIdx.x, BLOCK, nl, nu);
+        classes[3 * (int64_t)v] = nl;
+        classes[3 * (int64_t)v + 1] = nu;
+        classes[3 * (int64_t)v + 2] = verdict(nl, nu);
+    }
+}
+
+template <typename S>
+__global__ __launch_bounds__(64) void k_cp_fix_big(
+    const int32_t *__restrict__ off, const int32_t *__restrict__ cols,
+    const S *__restrict__ scal, const int32_t *__restrict__ fixbig,
+    int32_t *__restrict__ counters, uint32_t *__restrict__ classes) {
+    __shared__ uint16_t par[HEAVY_CAP];
+    __shared__ uint8_t lowf[HEAVY_CAP];
+    if (threadIdx.x != 0) return;
+    const int32_t n = counters[C_FIXBIG];
+    for (int32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const int32_t v = fixbig[i];
+        const int32_t b = off[v];
+        const int d = off[v + 1] - b;
+        if (d > HEAVY_CAP) {
+            atomicOr(counters + C_ERR, tg::ERRBIT_CAPACITY);
+            continue;
+        }
+        uint32_t nl, nu;
+        link_components(v, off, cols, scal, d, cols + b, par, 1, lowf, 1, nl, nu);
+        classes[3 * (int64_t)v] = nl;
+        classes[3 * (int64_t)v + 1] = nu;
+        classes[3 * (int64_t)v + 2] = verdict(nl, nu);
+    }
+}
+
+template <typename S>
+int launch_vv_cp_typed(tg::VvCpBuffers &b, const tg::VvCpConfig &cfg,
+                       bool classify, hipStream_t s) {
+    const int32_t V = (int32_t)b.n_points;
+    const S *scal = (const S *)b.scalars;
+    const int4 *tv4 = (const int4 *)b.tv;
+    const int cls = classify ? 1 : 0;
+    if (V > 0) {
+#define TG_LANE(CAP, BLK)                                                          \
+    hipLaunchKernelGGL((k_vv_cp_lane<CAP, BLK, S>), dim3(tg::grid_for(V, BLK)),      \
+                       dim3(BLK), 0, s, tv4, b.vt_off, b.vtc, scal, V, b.scratch,    \
+                       b.deg, b.classes, b.heavy, b.fix, b.counters, cls)
+        if (cfg.lane_cap <= 32) {
+            if (cfg.lane_block <= 64) TG_LANE(32, 64);
+            else if (cfg.lane_block <= 128) TG_LANE(32, 128);
+            else TG_LANE(32, 256);
+        } else {
+            if (cfg.lane_block <= 64) TG_LANE(64, 64);
+            else TG_LANE(64, 128);
+        }
+#undef TG_LANE
+        TG_LAUNCH_CHECK("k_vv_cp_lane");
+        hipLaunchKernelGGL((k_vv_cp_heavy<S>), dim3(256), dim3(64), 0, s, tv4, b.vt_off,
+                           b.vtc, scal, b.scratch, b.deg, b.classes, b.heavy, b.fix,
+                           b.fixbig, b.counters, cls);
+        TG_LAUNCH_CHECK("k_vv_cp_heavy");
+    }
+    int rc = tg::scan_exclusive_i32(b.deg, b.vv_off, V, b.scan_tmp, s);
+    if (rc) return rc;
+    if (V > 0) {
+        hipLaunchKernelGGL(k_vv_compact, dim3(tg::grid_for(V, 256)), dim3(256), 0, s,
+                           b.vt_off, b.vv_off, b.scratch, V, b.vv_cols);
+        TG_LAUNCH_CHECK("k_vv_compact");
+        if (classify) {
+            const unsigned g = std::min<unsigned>(tg::grid_for(V, 128), 2048u);
+            hipLaunchKernelGGL((k_cp_fix_lane<128, S>), dim3(g), dim3(128), 0, s, b.vv_off,
+                               b.vv_cols, scal, b.fix, b.fixbig, b.counters, b.classes);
+            TG_LAUNCH_CHECK("k_cp_fix_lane");
+            hipLaunchKernelGGL((k_cp_fix_big<S>), dim3(256), dim3(64), 0, s, b.vv_off,
+                               b.vv_cols, scal, b.fixbig, b.counters, b.classes);
+            TG_LAUNCH_CHECK("k_cp_fix_big");
+        }
+    }
+    return tg::OK;
+}
+
+}  // namespace
+
+namespace tg {
+
+int launch_vt_corners(VvCpBuffers &b, int block, hipStream_t s) {
+    const int64_t V = b.n_points, T = b.n_cells;
+    int rc = fill_i32(b.vt_cnt, 0, V, s);
+    if (!rc) rc = fill_i32(b.vt_cur, 0, V, s);
+    if (rc) return rc;
+    const int4 *tv4 = (const int4 *)b.tv;
+    unsigned g = grid_for(T, block);
+    if (g > 65536) g = 65536;
+    if (T > 0) {
+        hipLaunchKernelGGL(k_count_corners, dim3(g), dim3(block), 0, s, tv4, T, b.vt_cnt);
+        TG_LAUNCH_CHECK("k_count_corners");
+    }
+    rc = scan_exclusive_i32(b.vt_cnt, b.vt_off, V, b.scan_tmp, s);
+    if (rc) return rc;
+    if (T > 0) {
+        hipLaunchKernelGGL(k_fill_corners, dim3(g), dim3(block), 0, s, tv4, T, b.vt_off,
+                           b.vt_cur, b.vtc);
+        TG_LAUNCH_CHECK("k_fill_corners");
+    }
+    return OK;
+}
+
+int launch_sort_rows_i32(const int32_t *off, int32_t *vals, int64_t rows,
+                         hipStream_t s) {
+    if (rows <= 0) return OK;
+    unsigned g = grid_for(rows, 256);
+    if (g > 65536) g = 65536;
+    hipLaunchKernelGGL(k_sort_rows, dim3(g), dim3(256), 0, s, off, vals, rows);
+    TG_LAUNCH_CHECK("k_sort_rows");
+    return OK;
+}
+
+int launch_vv_cp(VvCpBuffers &b, const VvCpConfig &cfg, bool classify,
+                 hipStream_t s) {
+    return b.scalars_f64 ? launch_vv_cp_typed<double>(b, cfg, classify, s)
+                         : launch_vv_cp_typed<float>(b, cfg, classify, s);
+}
+
+int run_vv_cp(VvCpBuffers &b, const VvCpConfig &cfg, bool classify,
+              hipStream_t s) {
+    int rc = fill_i32(b.counters, 0, N_COUNTERS, s);
+    if (!rc) rc = launch_vt_corners(b, cfg.corner_block, s);
+    if (!rc) rc = launch_vv_cp(b, cfg, classify, s);
+    return rc;
+}
+
+}  // namespace tg
