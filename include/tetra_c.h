@@ -87,10 +87,14 @@ int tg_mesh_stream(tg_mesh *m, void **stream);
 int tg_mesh_counts(tg_mesh *m, int64_t *n_points, int64_t *n_cells,
                    int64_t *n_edges, int64_t *n_faces);
 
-/* Kernel shape of the VV + critical point pipeline: neighbours held per lane
- * (32 or 64) and lanes per workgroup (64, 128 or 256).  Chosen by the
- * autotuner per size class; any supported value gives identical results. */
-int tg_mesh_set_config(tg_mesh *m, int32_t lane_cap, int32_t lane_block);
+/* Kernel form and shape of the VV + critical point pipeline.  variant:
+ * 0 automatic, 1 vertex-lane form (vertex -> tet lists, one lane per
+ * vertex), 2 bucketed form (corners routed to 128/256-vertex buckets, one
+ * workgroup per bucket).  lane_cap (32 or 64) and lane_block (64, 128 or
+ * 256) shape the vertex-lane form.  Chosen by the autotuner per size class;
+ * every supported setting gives identical results. */
+int tg_mesh_set_config(tg_mesh *m, int32_t variant, int32_t lane_cap,
+                       int32_t lane_block);
 
 /* Build the relations whose TG_FLAG bits are set, plus what they need.
  * Bit 0 (validate) is ignored.  Asynchronous. */
@@ -129,6 +133,14 @@ int tg_labels6_copy(tg_mesh *m, uint8_t *labels);
  * padded VV width (src/cuda_extraction.cu:1154-1178), computed on the
  * device for a built mesh. */
 int tg_approx_max_vv(tg_mesh *m, int32_t *max_vv);
+
+/* Stage timing of the last pipeline run: when enabled, HIP events are
+ * recorded between the pipeline's launches on the mesh stream.
+ * tg_profile_read synchronises and returns up to max_stages durations in ms
+ * with their stage names (32 chars each) in launch order. */
+int tg_profile_enable(tg_mesh *m, int on);
+int tg_profile_read(tg_mesh *m, int32_t max_stages, int32_t *n_stages,
+                    float *ms, char *names);
 
 /* Free a device pointer handed out by the library to the caller. */
 int tg_device_free(void *p);

@@ -11,10 +11,15 @@ from oracle import binding as O
 pytestmark = pytest.mark.gpu
 
 
-def run_device(t, cells, n_points, scalars, cfg=None, twice=False):
+VARIANTS = [1, 2]  # vertex-lane form, bucketed form
+
+
+def run_device(t, cells, n_points, scalars, cfg=None, twice=False, variant=0):
     with t.Mesh(cells, n_points, scalars) as m:
         if cfg:
-            m.set_config(*cfg)
+            m.set_config(*cfg, variant=1)
+        elif variant:
+            m.set_config(variant=variant)
         m.vv_critpoints()
         if twice:
             m.vv_critpoints()
@@ -22,9 +27,9 @@ def run_device(t, cells, n_points, scalars, cfg=None, twice=False):
         return off, vals[:, 0].copy(), m.classes()
 
 
-def check(t, cells, n_points, scalars, cfg=None, twice=False):
-    print(f"case V={n_points} T={len(cells)} cfg={cfg}", flush=True)
-    off, cols, cls = run_device(t, cells, n_points, scalars, cfg, twice)
+def check(t, cells, n_points, scalars, cfg=None, twice=False, variant=0):
+    print(f"case V={n_points} T={len(cells)} cfg={cfg} variant={variant}", flush=True)
+    off, cols, cls = run_device(t, cells, n_points, scalars, cfg, twice, variant)
     VV = O.vv(cells, n_points).sorted_rows()
     assert np.array_equal(off, VV.off), "VV offsets differ"
     assert np.array_equal(cols, VV.val[:, 0]), "VV rows differ"
@@ -34,9 +39,10 @@ def check(t, cells, n_points, scalars, cfg=None, twice=False):
     return cls
 
 
-def test_config1_kuhn24(gpu):
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_config1_kuhn24(gpu, variant):
     m = gpu.kuhn(24, seed=1234)
-    cls = check(gpu, m.cells, m.n_points, m.scalars)
+    cls = check(gpu, m.cells, m.n_points, m.scalars, variant=variant)
     # uniform noise: every class occurs
     assert set(np.unique(cls[:, 2])) == {1, 2, 3, 4}
 
@@ -47,10 +53,11 @@ def test_lane_shapes_identical(gpu, cfg):
     check(gpu, m.cells, m.n_points, m.scalars, cfg=cfg)
 
 
+@pytest.mark.parametrize("variant", VARIANTS)
 @pytest.mark.parametrize("order", ["path", "shuffled"])
-def test_unsorted_cells(gpu, order):
+def test_unsorted_cells(gpu, order, variant):
     m = gpu.kuhn(10, seed=3, order=order)
-    check(gpu, m.cells, m.n_points, m.scalars)
+    check(gpu, m.cells, m.n_points, m.scalars, variant=variant)
 
 
 def test_irregular_hubs_over_64_neighbours(gpu):
@@ -58,51 +65,60 @@ def test_irregular_hubs_over_64_neighbours(gpu):
     m = gpu.kuhn(12, seed=11, split_fraction=0.15, n_hubs=6, hub_splits=60, scalar="sines")
     VV = O.vv(m.cells, m.n_points)
     assert np.diff(VV.off).max() > 64
-    check(gpu, m.cells, m.n_points, m.scalars)
+    check(gpu, m.cells, m.n_points, m.scalars, variant=1)
+    check(gpu, m.cells, m.n_points, m.scalars, variant=2)
     check(gpu, m.cells, m.n_points, m.scalars, cfg=(64, 128))
 
 
-def test_all_ties_constant_field(gpu):
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_all_ties_constant_field(gpu, variant):
     m = gpu.kuhn(8, seed=5, scalar="constant")
-    check(gpu, m.cells, m.n_points, m.scalars)
+    check(gpu, m.cells, m.n_points, m.scalars, variant=variant)
 
 
-def test_linear_field_one_min_one_max(gpu):
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_linear_field_one_min_one_max(gpu, variant):
     m = gpu.kuhn(9, permute_vertices=False, permute_cells=True, jitter=False, scalar="linear")
-    cls = check(gpu, m.cells, m.n_points, m.scalars)
+    cls = check(gpu, m.cells, m.n_points, m.scalars, variant=variant)
     assert (cls[:, 2] == 1).sum() == 1 and (cls[:, 2] == 2).sum() == 1
     assert (cls[:, 2] == 3).sum() == m.n_points - 2
 
 
-def test_gaussian_field_f64_scalars(gpu):
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_gaussian_field_f64_scalars(gpu, variant):
     m = gpu.kuhn(14, seed=21, scalar="gaussians")
-    check(gpu, m.cells, m.n_points, m.scalars.astype(np.float64) * (1 + 1e-9))
+    check(gpu, m.cells, m.n_points, m.scalars.astype(np.float64) * (1 + 1e-9), variant=variant)
 
 
-def test_nan_scalars(gpu):
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_nan_scalars(gpu, variant):
     m = gpu.kuhn(6, seed=2)
     s = m.scalars.copy()
     s[::7] = np.nan
-    check(gpu, m.cells, m.n_points, s)
+    check(gpu, m.cells, m.n_points, s, variant=variant)
 
 
-def test_isolated_and_tiny(gpu):
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_isolated_and_tiny(gpu, variant):
     # one tet plus three vertices no cell references: (0, 0, 4)
     cells = np.array([[0, 2, 4, 5]], np.int32)
-    cls = check(gpu, cells, 8, np.arange(8, dtype=np.float32))
+    cls = check(gpu, cells, 8, np.arange(8, dtype=np.float32), variant=variant)
     assert cls[1].tolist() == [0, 0, 4] and cls[3].tolist() == [0, 0, 4]
 
 
-def test_empty_mesh(gpu):
-    off, cols, cls = run_device(gpu, np.zeros((0, 4), np.int32), 5, np.zeros(5, np.float32))
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_empty_mesh(gpu, variant):
+    off, cols, cls = run_device(gpu, np.zeros((0, 4), np.int32), 5, np.zeros(5, np.float32),
+                                variant=variant)
     assert off.tolist() == [0] * 6 and cols.size == 0
     assert cls.tolist() == [[0, 0, 4]] * 5
 
 
-def test_repeat_is_deterministic(gpu):
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_repeat_is_deterministic(gpu, variant):
     m = gpu.kuhn(16, seed=9)
-    a = run_device(gpu, m.cells, m.n_points, m.scalars)
-    b = run_device(gpu, m.cells, m.n_points, m.scalars, twice=True)
+    a = run_device(gpu, m.cells, m.n_points, m.scalars, variant=variant)
+    b = run_device(gpu, m.cells, m.n_points, m.scalars, twice=True, variant=variant)
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
 
@@ -131,3 +147,18 @@ def test_labels6_match_oracle(gpu):
         lab = d.labels6()
         cls = d.classes()
     assert np.array_equal(lab, O.label6(cls))
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_hub_vertices_bucket_overflow(gpu, variant):
+    # many vertices above 32 neighbours: the bucketed form routes them to its
+    # serial kernel; both forms must agree with the oracle
+    m = gpu.kuhn(10, seed=17, split_fraction=0.3, n_hubs=40, hub_splits=40, scalar="gaussians")
+    check(gpu, m.cells, m.n_points, m.scalars, variant=variant)
+
+
+def test_bucket_boundaries_partial_last_bucket(gpu):
+    # V not a multiple of the bucket size, unreferenced trailing vertices
+    m = gpu.kuhn(7, seed=23)
+    check(gpu, m.cells, m.n_points + 77, np.concatenate([m.scalars, np.zeros(77, np.float32)]),
+          variant=2)

@@ -38,7 +38,7 @@ _HIP_SIGS = {
     "tg_mesh_stream": (C.c_int, [P, C.POINTER(P)]),
     "tg_mesh_counts": (C.c_int, [P, C.POINTER(I64), C.POINTER(I64), C.POINTER(I64),
                                  C.POINTER(I64)]),
-    "tg_mesh_set_config": (C.c_int, [P, I32, I32]),
+    "tg_mesh_set_config": (C.c_int, [P, I32, I32, I32]),
     "tg_build": (C.c_int, [P, U32]),
     "tg_critpoints": (C.c_int, [P]),
     "tg_vv_critpoints": (C.c_int, [P]),
@@ -52,6 +52,8 @@ _HIP_SIGS = {
     "tg_labels6_copy": (C.c_int, [P, P]),
     "tg_approx_max_vv": (C.c_int, [P, C.POINTER(I32)]),
     "tg_device_free": (C.c_int, [P]),
+    "tg_profile_enable": (C.c_int, [P, C.c_int]),
+    "tg_profile_read": (C.c_int, [P, I32, C.POINTER(I32), P, P]),
 }
 
 _HOST_SIGS = {

@@ -1,0 +1,37 @@
+// Bucketed VV + critical point pipeline (vv_cp_bucket.hip).
+#pragma once
+
+#include "pipeline.h"
+
+namespace tg {
+
+struct BucketPlan {
+    bool ok = false;      // mesh fits the bucketed form (V <= 2^24)
+    int bv_log = 7;       // vertices per bucket = 2^bv_log (7 or 8)
+    int32_t nb = 1;       // buckets
+    int fs = 0;           // buckets per coarse bin = 2^fs
+    int32_t nc = 1;       // coarse bins (<= 256)
+    int32_t nchunks = 1;  // tet chunks of the first routing level
+    int64_t tpc = 8192;   // tets per chunk
+    int32_t slices = 1;   // slices per coarse bin in the second level
+};
+
+// Record arrays are 3 x 4T uint32 (structure of arrays).  rec1 aliases the
+// padded row scratch and rec2 the CSR columns: both are dead while the
+// records live.
+struct BucketBuffers {
+    VvCpBuffers base;
+    BucketPlan plan;
+    int32_t *h1 = nullptr;      // [nc*nchunks + 1] level-1 counts -> offsets
+    int32_t *h2 = nullptr;      // [nb*slices + 1] level-2 counts -> offsets
+    uint32_t *rec1 = nullptr;   // level-1 output
+    uint32_t *rec2 = nullptr;   // level-2 output (bucket order)
+    int32_t *rowsrc = nullptr;  // [V] row start in scratch
+};
+
+BucketPlan plan_buckets(int64_t n_cells, int64_t n_points, int bv_log = 7);
+int launch_bucketed_vv_cp(BucketBuffers &k, bool classify, hipStream_t s);
+// Whole bucketed pipeline: counters, routing, VV rows, CSR, link pass.
+int run_vv_cp_bucketed(BucketBuffers &k, bool classify, hipStream_t s);
+
+}  // namespace tg

@@ -228,13 +228,15 @@ int tg_mesh_stream(tg_mesh *m, void **stream) {
     return TG_OK;
 }
 
-int tg_mesh_set_config(tg_mesh *m, int32_t lane_cap, int32_t lane_block) {
+int tg_mesh_set_config(tg_mesh *m, int32_t variant, int32_t lane_cap, int32_t lane_block) {
     if (!m) return fail(TG_ERR_ARG, "NULL mesh");
+    if (variant < 0 || variant > 2) return fail(TG_ERR_ARG, "variant must be 0, 1 or 2");
     if (lane_cap != 32 && lane_cap != 64) return fail(TG_ERR_ARG, "lane_cap must be 32 or 64");
     if (lane_block != 64 && lane_block != 128 && lane_block != 256)
         return fail(TG_ERR_ARG, "lane_block must be 64, 128 or 256");
     if (lane_cap == 64 && lane_block == 256)
         return fail(TG_ERR_ARG, "lane_cap 64 supports lane_block 64 or 128");
+    m->cfg.variant = variant;
     m->cfg.lane_cap = lane_cap;
     m->cfg.lane_block = lane_block;
     return TG_OK;
@@ -387,6 +389,32 @@ int tg_approx_max_vv(tg_mesh *m, int32_t *max_vv) {
     TG_HIP(hipStreamSynchronize(m->stream));
     mx *= 3;
     *max_vv = ((mx + 31) / 32) * 32;
+    return TG_OK;
+}
+
+int tg_profile_enable(tg_mesh *m, int on) {
+    if (!m) return fail(TG_ERR_ARG, "NULL mesh");
+    m->marks.on = on != 0;
+    m->marks.n = 0;
+    return TG_OK;
+}
+
+int tg_profile_read(tg_mesh *m, int32_t max_stages, int32_t *n_stages, float *ms,
+                    char *names) {
+    if (!m || !n_stages) return fail(TG_ERR_ARG, "NULL argument");
+    int rc = tg::sync_and_check(*m);
+    if (rc) return rc;
+    const int n = m->marks.n > 0 ? m->marks.n - 1 : 0;
+    *n_stages = n;
+    for (int i = 0; i < n && i < max_stages; i++) {
+        float t = 0;
+        TG_HIP(hipEventElapsedTime(&t, m->marks.ev[i], m->marks.ev[i + 1]));
+        if (ms) ms[i] = t;
+        if (names) {
+            std::strncpy(names + 32 * i, m->marks.name[i + 1], 31);
+            names[32 * i + 31] = 0;
+        }
+    }
     return TG_OK;
 }
 

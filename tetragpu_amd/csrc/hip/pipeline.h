@@ -22,8 +22,21 @@ enum ErrBit : int { ERRBIT_CAPACITY = 1 };
 // Largest vertex degree the serial per-vertex kernels hold in LDS.
 constexpr int HEAVY_CAP = 4096;
 
+// Optional per-stage HIP events recorded between the launches of a pipeline
+// (enabled by tg_profile_enable); mark() is a no-op when disabled.
+struct StageMarks {
+    static constexpr int MAX = 32;
+    bool on = false;
+    int n = 0;
+    hipEvent_t ev[MAX] = {};
+    const char *name[MAX] = {};
+    void begin(hipStream_t s);
+    void mark(const char *stage, hipStream_t s);
+};
+
 // Tunables of the VV + critical point pipeline (picked by the autotuner).
 struct VvCpConfig {
+    int variant = 0;       // 0 auto, 1 vertex-lane form, 2 bucketed form
     int lane_cap = 32;     // neighbours held per lane (32 or 64)
     int lane_block = 128;  // lanes (vertices) per workgroup
     int corner_block = 256;
@@ -47,6 +60,7 @@ struct VvCpBuffers {
     int32_t *heavy = nullptr, *fix = nullptr, *fixbig = nullptr,
             *counters = nullptr;
     void *scan_tmp = nullptr;
+    StageMarks *marks = nullptr;
 };
 
 // vertex -> corner incidence (p = 4t + j) grouped by vertex, unordered rows.
@@ -57,6 +71,11 @@ int launch_sort_rows_i32(const int32_t *off, int32_t *vals, int64_t rows,
 // VV CSR (ascending rows) + classes[3V]; assumes launch_vt_corners ran.
 int launch_vv_cp(VvCpBuffers &b, const VvCpConfig &cfg, bool classify,
                  hipStream_t s);
+// Gather rows from padded scratch (row of v starts at rowsrc[v]) into the CSR.
+int launch_rows_compact(const int32_t *rowsrc, const int32_t *vv_off, const int32_t *scratch,
+                        int32_t V, int32_t *cols, hipStream_t s);
+// Row-membership link components for the vertices queued in fix/fixbig.
+int launch_cp_fix(VvCpBuffers &b, hipStream_t s);
 // Whole pipeline: counters reset, VT corners, VV, critical points.
 int run_vv_cp(VvCpBuffers &b, const VvCpConfig &cfg, bool classify,
               hipStream_t s);

@@ -1,7 +1,7 @@
 // The mesh handle behind tg_mesh* and the relation builders it dispatches.
 #pragma once
 
-#include "pipeline.h"
+#include "bucket.h"
 
 namespace tg {
 
@@ -40,6 +40,9 @@ struct tg_mesh {
     // VV + critical point pipeline
     tg::DevArr vt_cnt, vt_off, vt_cur, vtc, scratch, deg, vv_off, vv_cols, classes,
         heavy, fix, fixbig, counters, scan_tmp, labels;
+    tg::DevArr bk_h1, bk_h2, bk_rowsrc;
+    tg::BucketPlan bplan;
+    bool vt_ready = false, bk_ready = false;
     bool hot_ready = false;         // buffers sized for the hot path
     bool vt_corners_built = false;  // vt_off / vtc valid
     bool vt_sorted = false;         // vtc rows ascending
@@ -58,14 +61,20 @@ struct tg_mesh {
     int64_t tt_nnz = 0, ff_nnz = 0, ee_nnz = 0, ef_nnz = 0;
 
     void *h_pinned = nullptr;  // small pinned staging area (64 int64)
+    tg::StageMarks marks;      // per-stage events of the last pipeline run
 
     void release_all();
 };
 
 namespace tg {
 
-// Allocate (once) every buffer the VV + critical point pipeline touches.
+// Allocate (once) the buffers shared by both VV forms, the vertex -> corner
+// arrays, and the bucketed form's arrays.
 int prepare_hot_path(tg_mesh &m);
+int prepare_vt(tg_mesh &m);
+int prepare_buckets(tg_mesh &m);
+// Variant the hot path runs for this mesh (1 vertex-lane, 2 bucketed).
+int hot_variant(const tg_mesh &m);
 // Enqueue VV + classes (recomputed from TV and scalars).
 int run_hot_path(tg_mesh &m);
 int ensure_vt_corners(tg_mesh &m);

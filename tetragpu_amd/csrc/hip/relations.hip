@@ -42,8 +42,13 @@ void tg_mesh::release_all() {
                          &et_off,   &et_vals,  &face_off, &vf_vals,  &fv,       &tf,
                          &fe,       &ft,       &ef_off,   &ef_vals,  &tt_off,   &tt_vals,
                          &ff_off,   &ff_vals,  &ee_off,   &ee_vals,  &bitmap,   &rank_tmp,
-                         &tmp_a,    &tmp_b,    &tmp_c};
+                         &tmp_a,    &tmp_b,    &tmp_c,    &bk_h1,    &bk_h2,    &bk_rowsrc};
     for (auto *a : all) a->release();
+    for (auto &e : marks.ev)
+        if (e) {
+            (void)hipEventDestroy(e);
+            e = nullptr;
+        }
 }
 
 namespace tg {
@@ -69,6 +74,7 @@ VvCpBuffers hot_buffers(tg_mesh &m) {
     b.fixbig = m.fixbig.as<int32_t>();
     b.counters = m.counters.as<int32_t>();
     b.scan_tmp = m.scan_tmp.p;
+    b.marks = &m.marks;
     return b;
 }
 
@@ -76,36 +82,90 @@ int prepare_hot_path(tg_mesh &m) {
     if (m.hot_ready) return OK;
     const size_t V = (size_t)m.n_points, T = (size_t)m.n_cells, I = sizeof(int32_t);
     int rc;
-    if ((rc = m.vt_cnt.ensure(I * V)) || (rc = m.vt_off.ensure(I * (V + 1))) ||
-        (rc = m.vt_cur.ensure(I * V)) || (rc = m.vtc.ensure(I * 4 * T)) ||
-        (rc = m.scratch.ensure(I * 12 * T)) || (rc = m.deg.ensure(I * V)) ||
+    if ((rc = m.scratch.ensure(I * 12 * T)) || (rc = m.deg.ensure(I * V)) ||
         (rc = m.vv_off.ensure(I * (V + 1))) || (rc = m.vv_cols.ensure(I * 12 * T)) ||
         (rc = m.classes.ensure(I * 3 * V)) || (rc = m.heavy.ensure(I * V)) ||
         (rc = m.fix.ensure(I * V)) || (rc = m.fixbig.ensure(I * V)) ||
         (rc = m.counters.ensure(I * N_COUNTERS)) ||
         (rc = m.scan_tmp.ensure(scan_tmp_bytes(std::max<int64_t>(m.n_points, 12 * m.n_cells)))))
         return rc;
+    m.bplan = plan_buckets(m.n_cells, m.n_points);
     m.hot_ready = true;
     return OK;
 }
 
-int run_hot_path(tg_mesh &m) {
+int prepare_vt(tg_mesh &m) {
+    if (m.vt_ready) return OK;
     int rc = prepare_hot_path(m);
     if (rc) return rc;
-    VvCpBuffers b = hot_buffers(m);
-    rc = run_vv_cp(b, m.cfg, true, m.stream);
+    const size_t V = (size_t)m.n_points, T = (size_t)m.n_cells, I = sizeof(int32_t);
+    if ((rc = m.vt_cnt.ensure(I * V)) || (rc = m.vt_off.ensure(I * (V + 1))) ||
+        (rc = m.vt_cur.ensure(I * V)) || (rc = m.vtc.ensure(I * 4 * T)))
+        return rc;
+    m.vt_ready = true;
+    return OK;
+}
+
+int prepare_buckets(tg_mesh &m) {
+    if (m.bk_ready) return OK;
+    int rc = prepare_hot_path(m);
     if (rc) return rc;
-    m.vt_corners_built = true;
-    m.vt_sorted = false;
+    const BucketPlan &p = m.bplan;
+    const size_t I = sizeof(int32_t);
+    const size_t n1 = (size_t)p.nc * (size_t)p.nchunks + 1, n2 = (size_t)p.nb * p.slices + 1;
+    if ((rc = m.bk_h1.ensure(I * n1)) || (rc = m.bk_h2.ensure(I * n2)) ||
+        (rc = m.bk_rowsrc.ensure(I * (size_t)m.n_points)) ||
+        (rc = m.scan_tmp.ensure(scan_tmp_bytes((int64_t)std::max(
+             {n1, n2, (size_t)m.n_points, (size_t)(12 * m.n_cells)})))))
+        return rc;
+    m.bk_ready = true;
+    return OK;
+}
+
+int hot_variant(const tg_mesh &m) {
+    if (m.cfg.variant == 1) return 1;
+    if (m.cfg.variant == 2 && m.bplan.ok) return 2;
+    return m.bplan.ok ? 2 : 1;
+}
+
+static BucketBuffers bucket_buffers(tg_mesh &m) {
+    BucketBuffers k;
+    k.base = hot_buffers(m);
+    k.plan = m.bplan;
+    k.h1 = m.bk_h1.as<int32_t>();
+    k.h2 = m.bk_h2.as<int32_t>();
+    k.rec1 = m.scratch.as<uint32_t>();
+    k.rec2 = m.vv_cols.as<uint32_t>();
+    k.rowsrc = m.bk_rowsrc.as<int32_t>();
+    return k;
+}
+
+// Enqueue VV (and classes when classify) in the mesh's chosen form.
+static int run_vv(tg_mesh &m, bool classify) {
+    int rc = prepare_hot_path(m);
+    if (rc) return rc;
+    if (hot_variant(m) == 2) {
+        if ((rc = prepare_buckets(m))) return rc;
+        BucketBuffers k = bucket_buffers(m);
+        if ((rc = run_vv_cp_bucketed(k, classify, m.stream))) return rc;
+    } else {
+        if ((rc = prepare_vt(m))) return rc;
+        VvCpBuffers b = hot_buffers(m);
+        if ((rc = run_vv_cp(b, m.cfg, classify, m.stream))) return rc;
+        m.vt_corners_built = true;
+        m.vt_sorted = false;
+    }
     m.vv_built = true;
-    m.cp_built = true;
+    if (classify) m.cp_built = true;
     m.counters_pending = true;
     return OK;
 }
 
+int run_hot_path(tg_mesh &m) { return run_vv(m, true); }
+
 int ensure_vt_corners(tg_mesh &m) {
     if (m.vt_corners_built) return OK;
-    int rc = prepare_hot_path(m);
+    int rc = prepare_vt(m);
     if (rc) return rc;
     VvCpBuffers b = hot_buffers(m);
     if ((rc = launch_vt_corners(b, m.cfg.corner_block, m.stream))) return rc;
@@ -127,15 +187,7 @@ int ensure_vt_sorted(tg_mesh &m) {
 
 int ensure_vv(tg_mesh &m) {
     if (m.vv_built) return OK;
-    int rc = prepare_hot_path(m);
-    if (rc) return rc;
-    if ((rc = fill_i32(m.counters.as<int32_t>(), 0, N_COUNTERS, m.stream))) return rc;
-    if ((rc = ensure_vt_corners(m))) return rc;
-    VvCpBuffers b = hot_buffers(m);
-    if ((rc = launch_vv_cp(b, m.cfg, false, m.stream))) return rc;
-    m.vv_built = true;
-    m.counters_pending = true;
-    return OK;
+    return run_vv(m, false);
 }
 
 static int ensure_vt(tg_mesh &m) {

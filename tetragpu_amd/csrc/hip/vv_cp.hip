@@ -20,7 +20,7 @@
 //      whose shared-tet components number 0 or 1 already has the reference
 //      count; only vertices with >= 2 components in a class are recomputed
 //      with the row-membership rule on the finished VV rows.
-#include "pipeline.h"
+#include "bucket.h"
 
 #include <algorithm>
 
@@ -364,6 +364,30 @@ __global__ __launch_bounds__(256) void k_vv_compact(
     }
 }
 
+__global__ __launch_bounds__(256) void k_rows_compact(
+    const int32_t *__restrict__ rowsrc, const int32_t *__restrict__ vv_off,
+    const int32_t *__restrict__ scratch, int32_t V, int32_t *__restrict__ cols) {
+    __shared__ int32_t s_off[257];
+    __shared__ int32_t s_src[256];
+    const int32_t v0 = (int32_t)blockIdx.x * 256;
+    const int nv = min(256, V - v0);
+    for (int i = threadIdx.x; i <= nv; i += 256) {
+        s_off[i] = vv_off[v0 + i];
+        if (i < nv) s_src[i] = rowsrc[v0 + i];
+    }
+    __syncthreads();
+    const int32_t ob = s_off[0], oe = s_off[nv];
+    for (int32_t o = ob + threadIdx.x; o < oe; o += 256) {
+        int lo = 0, hi = nv - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_off[mid] <= o) lo = mid;
+            else hi = mid - 1;
+        }
+        cols[o] = scratch[(int64_t)s_src[lo] + (o - s_off[lo])];
+    }
+}
+
 // ---- row-membership link components for flagged vertices -------------------
 // Link edge (a, c) between two neighbours of v exists iff c is in VV(a)
 // (critPoints.cu:231-233).  Each undirected edge is found once, from the
@@ -430,30 +454,83 @@ __device__ void link_components(int32_t v, const int32_t *__restrict__ off,
         }
 }
 
-template <int BLOCK, typename S>
-__global__ __launch_bounds__(BLOCK) void k_cp_fix_lane(
+__device__ __forceinline__ uint32_t n_components64(const uint64_t *adj, uint64_t mask) {
+    uint32_t cnt = 0;
+    uint64_t rem = mask;
+    while (rem) {
+        uint64_t comp = rem & (0ull - rem);
+        uint64_t front = comp;
+        while (front) {
+            const int f = __ffsll((long long)front) - 1;
+            front &= front - 1ull;
+            const uint64_t nb = adj[f] & mask & ~comp;
+            comp |= nb;
+            front |= nb;
+        }
+        rem &= ~comp;
+        cnt++;
+    }
+    return cnt;
+}
+
+// One wave per queued vertex v (degree <= 64): lane i holds neighbour
+// a = VV(v)[i], walks VV(a) and marks each member of VV(v) it meets, giving
+// a 64-bit adjacency row of the link graph under the row-membership rule;
+// lane 0 then counts lower and upper components by bit-parallel BFS.
+template <typename S>
+__global__ __launch_bounds__(64) void k_cp_fix_wave(
     const int32_t *__restrict__ off, const int32_t *__restrict__ cols,
     const S *__restrict__ scal, const int32_t *__restrict__ fix,
     int32_t *__restrict__ fixbig, int32_t *__restrict__ counters,
     uint32_t *__restrict__ classes) {
-    __shared__ uint16_t s_par[64 * BLOCK];
-    __shared__ uint8_t s_low[64 * BLOCK];
+    __shared__ int32_t nbs[64];
+    __shared__ uint64_t adjm[64];
+    const int lane = threadIdx.x;
     const int32_t n = counters[C_FIX];
-    for (int32_t i = (int32_t)blockIdx.x * BLOCK + threadIdx.x; i < n;
-         i += (int32_t)gridDim.x * BLOCK) {
+    for (int32_t i = blockIdx.x; i < n; i += gridDim.x) {
         const int32_t v = fix[i];
         const int32_t b = off[v];
         const int d = off[v + 1] - b;
         if (d > 64) {
-            fixbig[atomicAdd(counters + C_FIXBIG, 1)] = v;
+            if (lane == 0) fixbig[atomicAdd(counters + C_FIXBIG, 1)] = v;
             continue;
         }
-        uint32_t nl, nu;
-        link_components(v, off, cols, scal, d, cols + b, s_par + threadIdx.x, BLOCK,
-                        s_low + threadIdx.x, BLOCK, nl, nu);
-        classes[3 * (int64_t)v] = nl;
-        classes[3 * (int64_t)v + 1] = nu;
-        classes[3 * (int64_t)v + 2] = verdict(nl, nu);
+        const bool on = lane < d;
+        const int32_t a = on ? cols[b + lane] : INT32_MAX;
+        const bool lo = on && is_lower(scal[a], scal[v], a, v);
+        const uint64_t lowmask = __ballot(lo);
+        const uint64_t full = d == 64 ? ~0ull : ((1ull << d) - 1ull);
+        nbs[lane] = a;
+        __syncthreads();
+        uint64_t row = 0;
+        if (on) {
+            const int32_t rb = off[a], re = off[a + 1];
+            for (int32_t j0 = rb; j0 < re; j0 += 8) {
+                int32_t c[8];
+#pragma unroll
+                for (int k = 0; k < 8; k++) c[k] = j0 + k < re ? cols[j0 + k] : INT32_MAX;
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    int lo2 = 0, hi2 = d;  // first index with nbs >= c
+                    while (lo2 < hi2) {
+                        const int mid = (lo2 + hi2) >> 1;
+                        if (nbs[mid] < c[k]) lo2 = mid + 1;
+                        else hi2 = mid;
+                    }
+                    if (lo2 < d && nbs[lo2] == c[k]) row |= 1ull << lo2;
+                }
+            }
+        }
+        adjm[lane] = row;
+        __syncthreads();
+        if (lane == 0) {
+            const uint32_t nl = n_components64(adjm, lowmask & full);
+            const uint32_t nu = n_components64(adjm, ~lowmask & full);
+            classes[3 * (int64_t)v] = nl;
+            classes[3 * (int64_t)v + 1] = nu;
+            classes[3 * (int64_t)v + 2] = verdict(nl, nu);
+        }
+        __syncthreads();
     }
 }
 
@@ -483,6 +560,21 @@ __global__ __launch_bounds__(64) void k_cp_fix_big(
 }
 
 template <typename S>
+int launch_cp_fix_typed(tg::VvCpBuffers &b, hipStream_t s) {
+    const int32_t V = (int32_t)b.n_points;
+    const S *scal = (const S *)b.scalars;
+    const unsigned g = std::min<unsigned>(tg::grid_for(V, 1), 16384u);
+    hipLaunchKernelGGL((k_cp_fix_wave<S>), dim3(g), dim3(64), 0, s, b.vv_off, b.vv_cols, scal,
+                       b.fix, b.fixbig, b.counters, b.classes);
+    TG_LAUNCH_CHECK("k_cp_fix_wave");
+    hipLaunchKernelGGL((k_cp_fix_big<S>), dim3(256), dim3(64), 0, s, b.vv_off, b.vv_cols, scal,
+                       b.fixbig, b.counters, b.classes);
+    TG_LAUNCH_CHECK("k_cp_fix_big");
+    if (b.marks) b.marks->mark("cp_fix", s);
+    return tg::OK;
+}
+
+template <typename S>
 int launch_vv_cp_typed(tg::VvCpBuffers &b, const tg::VvCpConfig &cfg,
                        bool classify, hipStream_t s) {
     const int32_t V = (int32_t)b.n_points;
@@ -504,25 +596,24 @@ int launch_vv_cp_typed(tg::VvCpBuffers &b, const tg::VvCpConfig &cfg,
         }
 #undef TG_LANE
         TG_LAUNCH_CHECK("k_vv_cp_lane");
+        if (b.marks) b.marks->mark("vv_lane", s);
         hipLaunchKernelGGL((k_vv_cp_heavy<S>), dim3(256), dim3(64), 0, s, tv4, b.vt_off,
                            b.vtc, scal, b.scratch, b.deg, b.classes, b.heavy, b.fix,
                            b.fixbig, b.counters, cls);
         TG_LAUNCH_CHECK("k_vv_cp_heavy");
+        if (b.marks) b.marks->mark("vv_heavy", s);
     }
     int rc = tg::scan_exclusive_i32(b.deg, b.vv_off, V, b.scan_tmp, s);
     if (rc) return rc;
+    if (b.marks) b.marks->mark("vv_scan", s);
     if (V > 0) {
         hipLaunchKernelGGL(k_vv_compact, dim3(tg::grid_for(V, 256)), dim3(256), 0, s,
                            b.vt_off, b.vv_off, b.scratch, V, b.vv_cols);
         TG_LAUNCH_CHECK("k_vv_compact");
+        if (b.marks) b.marks->mark("vv_compact", s);
         if (classify) {
-            const unsigned g = std::min<unsigned>(tg::grid_for(V, 128), 2048u);
-            hipLaunchKernelGGL((k_cp_fix_lane<128, S>), dim3(g), dim3(128), 0, s, b.vv_off,
-                               b.vv_cols, scal, b.fix, b.fixbig, b.counters, b.classes);
-            TG_LAUNCH_CHECK("k_cp_fix_lane");
-            hipLaunchKernelGGL((k_cp_fix_big<S>), dim3(256), dim3(64), 0, s, b.vv_off,
-                               b.vv_cols, scal, b.fixbig, b.counters, b.classes);
-            TG_LAUNCH_CHECK("k_cp_fix_big");
+            int rc2 = tg::launch_cp_fix(b, s);
+            if (rc2) return rc2;
         }
     }
     return tg::OK;
@@ -532,25 +623,72 @@ int launch_vv_cp_typed(tg::VvCpBuffers &b, const tg::VvCpConfig &cfg,
 
 namespace tg {
 
+void StageMarks::begin(hipStream_t s) {
+    n = 0;
+    if (!on) return;
+    mark("start", s);
+}
+
+void StageMarks::mark(const char *stage, hipStream_t s) {
+    if (!on || n >= MAX) return;
+    if (!ev[n] && hipEventCreate(&ev[n]) != hipSuccess) return;
+    name[n] = stage;
+    (void)hipEventRecord(ev[n], s);
+    n++;
+}
+
 int launch_vt_corners(VvCpBuffers &b, int block, hipStream_t s) {
     const int64_t V = b.n_points, T = b.n_cells;
     int rc = fill_i32(b.vt_cnt, 0, V, s);
     if (!rc) rc = fill_i32(b.vt_cur, 0, V, s);
     if (rc) return rc;
+    if (b.marks) b.marks->mark("vt_fill0", s);
     const int4 *tv4 = (const int4 *)b.tv;
     unsigned g = grid_for(T, block);
     if (g > 65536) g = 65536;
     if (T > 0) {
         hipLaunchKernelGGL(k_count_corners, dim3(g), dim3(block), 0, s, tv4, T, b.vt_cnt);
         TG_LAUNCH_CHECK("k_count_corners");
+        if (b.marks) b.marks->mark("vt_count", s);
     }
     rc = scan_exclusive_i32(b.vt_cnt, b.vt_off, V, b.scan_tmp, s);
     if (rc) return rc;
+    if (b.marks) b.marks->mark("vt_scan", s);
     if (T > 0) {
         hipLaunchKernelGGL(k_fill_corners, dim3(g), dim3(block), 0, s, tv4, T, b.vt_off,
                            b.vt_cur, b.vtc);
         TG_LAUNCH_CHECK("k_fill_corners");
+        if (b.marks) b.marks->mark("vt_scatter", s);
     }
+    return OK;
+}
+
+int launch_rows_compact(const int32_t *rowsrc, const int32_t *vv_off, const int32_t *scratch,
+                        int32_t V, int32_t *cols, hipStream_t s) {
+    if (V <= 0) return OK;
+    hipLaunchKernelGGL(k_rows_compact, dim3(grid_for(V, 256)), dim3(256), 0, s, rowsrc, vv_off,
+                       scratch, V, cols);
+    TG_LAUNCH_CHECK("k_rows_compact");
+    return OK;
+}
+
+int launch_cp_fix(VvCpBuffers &b, hipStream_t s) {
+    if (b.n_points <= 0) return OK;
+    return b.scalars_f64 ? launch_cp_fix_typed<double>(b, s) : launch_cp_fix_typed<float>(b, s);
+}
+
+int run_vv_cp_bucketed(BucketBuffers &k, bool classify, hipStream_t s) {
+    VvCpBuffers &b = k.base;
+    if (b.marks) b.marks->begin(s);
+    int rc = fill_i32(b.counters, 0, N_COUNTERS, s);
+    if (rc) return rc;
+    const int32_t V = (int32_t)b.n_points;
+    if (V > 0 && (rc = launch_bucketed_vv_cp(k, classify, s))) return rc;
+    if ((rc = scan_exclusive_i32(b.deg, b.vv_off, V, b.scan_tmp, s))) return rc;
+    if (b.marks) b.marks->mark("vv_scan", s);
+    if ((rc = launch_rows_compact(k.rowsrc, b.vv_off, b.scratch, V, b.vv_cols, s))) return rc;
+    if (b.marks) b.marks->mark("vv_compact", s);
+    if (classify && (rc = launch_cp_fix(b, s))) return rc;
     return OK;
 }
 
@@ -572,6 +710,7 @@ int launch_vv_cp(VvCpBuffers &b, const VvCpConfig &cfg, bool classify,
 
 int run_vv_cp(VvCpBuffers &b, const VvCpConfig &cfg, bool classify,
               hipStream_t s) {
+    if (b.marks) b.marks->begin(s);
     int rc = fill_i32(b.counters, 0, N_COUNTERS, s);
     if (!rc) rc = launch_vt_corners(b, cfg.corner_block, s);
     if (!rc) rc = launch_vv_cp(b, cfg, classify, s);

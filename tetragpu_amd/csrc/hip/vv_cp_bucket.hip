@@ -1,0 +1,696 @@
+// VV + critical points, bucketed form (the default hot path on gfx950).
+//
+// Vertices are grouped in buckets of BV = 2^BV_LOG consecutive ids.  Every
+// tet corner travels once, as a 12-byte record {v, o0, o1, o2} (four 24-bit
+// fields: the corner's vertex and the tet's other three vertices), to v's
+// bucket; then one workgroup per bucket builds the neighbour sets and link
+// components of all its vertices in LDS.
+//
+// Routing is two-level so that every workgroup writes to a few hundred
+// output lines at a time (which L2 merges) instead of to all buckets:
+//   L1  k_l1_count / k_l1_scatter : chunks of tets -> <= 256 coarse bins
+//       (a coarse bin = 2^FS consecutive buckets); per-(bin, chunk) counts
+//       h1[bin][chunk], exclusive scan, LDS-atomic ranks.
+//   L2  k_l2_count / k_l2_scatter : slices of each coarse bin -> buckets;
+//       per-(bucket, slice) counts h2[bucket][slice], exclusive scan (which
+//       also gives each bucket's record range), LDS-atomic ranks.
+//   B   k_bkt_vv_cp : per bucket, a lock-free linear-probing hash set of 32
+//       slots per vertex (neighbour ids), a 32-bit adjacency mask per slot
+//       from the shared-tet link triangles, lower/upper masks from the
+//       scalars (ballots), link components by bit-parallel BFS, rows sorted
+//       by a 32-register sorting network and staged in LDS for coalesced
+//       stores.  Vertices with more than 32 neighbours go to k_bkt_heavy.
+// Then: scan of degrees, row compaction into the CSR, and the row-membership
+// link pass for vertices whose shared-tet link has >= 2 components in a class
+// (the shared-tet link graph is a subgraph of the reference's, so counts of 0
+// or 1 are already final; critPoints.cu:206-246).
+#include <algorithm>
+
+#include "bucket.h"
+
+namespace {
+
+using tg::C_ERR;
+using tg::C_FIX;
+using tg::C_FIXBIG;
+using tg::C_HEAVY;
+using tg::C_MAXDEG;
+using tg::HEAVY_CAP;
+
+constexpr int SLOTS = 32;
+constexpr uint32_t M24 = 0xFFFFFFu;
+
+template <typename S>
+__device__ __forceinline__ bool is_lower(S sa, S sv, int32_t a, int32_t v) {
+    return (sa == sv) ? (a < v) : (sa < sv);
+}
+
+__device__ __forceinline__ uint32_t verdict(uint32_t nl, uint32_t nu) {
+    if (nl >= 1 && nu == 0) return 2u;
+    if (nl == 0 && nu >= 1) return 1u;
+    if (nl == 1 && nu == 1) return 3u;
+    return 4u;
+}
+
+__device__ __forceinline__ void wave_append(int32_t *list, int32_t *counter,
+                                            int32_t v, bool pred) {
+    const uint64_t mask = __ballot(pred);
+    if (!mask) return;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    int32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (int32_t)__popcll(mask));
+    base = __shfl(base, leader, 64);
+    if (pred) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = v;
+}
+
+// ---- corner records ------------------------------------------------------------
+struct Rec {
+    uint32_t *w0, *w1, *w2;  // structure of arrays, n records each
+};
+
+__device__ __forceinline__ void rec_store(const Rec &r, int64_t i, uint32_t v, uint32_t a,
+                                          uint32_t b, uint32_t c) {
+    r.w0[i] = v | (a << 24);
+    r.w1[i] = (a >> 8) | (b << 16);
+    r.w2[i] = (b >> 16) | (c << 8);
+}
+
+__device__ __forceinline__ void rec_unpack(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t &v,
+                                           uint32_t &a, uint32_t &b, uint32_t &c) {
+    v = w0 & M24;
+    a = (w0 >> 24) | ((w1 & 0xFFFFu) << 8);
+    b = (w1 >> 16) | ((w2 & 0xFFu) << 16);
+    c = w2 >> 8;
+}
+
+// ---- L1: chunks of tets -> coarse bins -------------------------------------
+constexpr int L1_THREADS = 1024;
+constexpr int MAX_COARSE = 256;
+constexpr int MAX_FINE = 512;
+
+__global__ __launch_bounds__(L1_THREADS) void k_l1_count(const int4 *__restrict__ tv4, int64_t T,
+                                                          int64_t tpc, int cshift, int32_t nc,
+                                                          int32_t nch, int32_t *__restrict__ h1) {
+    __shared__ int32_t h[MAX_COARSE];
+    for (int i = threadIdx.x; i < nc; i += L1_THREADS) h[i] = 0;
+    __syncthreads();
+    const int64_t t0 = (int64_t)blockIdx.x * tpc, t1 = min(T, t0 + tpc);
+    for (int64_t t = t0 + threadIdx.x; t < t1; t += L1_THREADS) {
+        const int4 q = tv4[t];
+        atomicAdd(&h[q.x >> cshift], 1);
+        atomicAdd(&h[q.y >> cshift], 1);
+        atomicAdd(&h[q.z >> cshift], 1);
+        atomicAdd(&h[q.w >> cshift], 1);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nc; i += L1_THREADS) h1[(int64_t)i * nch + blockIdx.x] = h[i];
+}
+
+// Staged scatter shared by both routing levels: a tile of records is
+// counting-sorted by destination bin in LDS, then each bin's run is written
+// with consecutive lanes to consecutive addresses.
+constexpr int STAGE = 4096;  // records per tile
+
+struct Stage {
+    uint32_t w0[STAGE], w1[STAGE], w2[STAGE];
+    int32_t cnt[MAX_FINE], toff[MAX_FINE], gbase[MAX_FINE];
+    int32_t total;
+};
+
+// Exclusive scan of cnt[0..nbin) into toff, total into st.total (block-wide).
+__device__ __forceinline__ void stage_scan(Stage &st, int nbin) {
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        int32_t carry = 0;
+        for (int b0 = 0; b0 < nbin; b0 += 64) {
+            const int i = b0 + lane;
+            const int32_t x = i < nbin ? st.cnt[i] : 0;
+            const int32_t inc = tgd::wave_inclusive_sum(x);
+            if (i < nbin) st.toff[i] = carry + inc - x;
+            carry += __shfl(inc, 63, 64);
+        }
+        if (lane == 0) st.total = carry;
+    }
+}
+
+// Write the staged tile: slot -> bin via binary search over toff.
+__device__ __forceinline__ void stage_flush(Stage &st, int nbin, Rec out, int nthreads) {
+    const int32_t total = st.total;
+    for (int slot = threadIdx.x; slot < total; slot += nthreads) {
+        int lo = 0, hi = nbin - 1;  // last bin with toff <= slot and cnt > 0
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (st.toff[mid] <= slot) lo = mid;
+            else hi = mid - 1;
+        }
+        while (lo > 0 && st.cnt[lo] == 0) lo--;  // never taken: the last bin with toff <= slot is non-empty
+        const int64_t pos = (int64_t)st.gbase[lo] + (slot - st.toff[lo]);
+        out.w0[pos] = st.w0[slot];
+        out.w1[pos] = st.w1[slot];
+        out.w2[pos] = st.w2[slot];
+    }
+}
+
+__global__ __launch_bounds__(L1_THREADS) void k_l1_scatter(const int4 *__restrict__ tv4, int64_t T,
+                                                            int64_t tpc, int cshift, int32_t nc,
+                                                            int32_t nch,
+                                                            const int32_t *__restrict__ h1s,
+                                                            Rec out) {
+    __shared__ Stage st;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < nc; i += L1_THREADS) st.gbase[i] = h1s[(int64_t)i * nch + blockIdx.x];
+    const int64_t t0 = (int64_t)blockIdx.x * tpc, t1 = min(T, t0 + tpc);
+    for (int64_t tb = t0; tb < t1; tb += STAGE / 4) {
+        for (int i = tid; i < nc; i += L1_THREADS) st.cnt[i] = 0;
+        __syncthreads();
+        const int64_t t = tb + tid;
+        const bool on = tid < STAGE / 4 && t < t1;
+        int4 q = make_int4(0, 0, 0, 0);
+        int r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+        if (on) {
+            q = tv4[t];
+            r0 = atomicAdd(&st.cnt[q.x >> cshift], 1);
+            r1 = atomicAdd(&st.cnt[q.y >> cshift], 1);
+            r2 = atomicAdd(&st.cnt[q.z >> cshift], 1);
+            r3 = atomicAdd(&st.cnt[q.w >> cshift], 1);
+        }
+        __syncthreads();
+        stage_scan(st, nc);
+        __syncthreads();
+        if (on) {
+            const uint32_t x = q.x, y = q.y, z = q.z, w = q.w;
+            auto put = [&](int slot, uint32_t v, uint32_t a, uint32_t b, uint32_t c) {
+                st.w0[slot] = v | (a << 24);
+                st.w1[slot] = (a >> 8) | (b << 16);
+                st.w2[slot] = (b >> 16) | (c << 8);
+            };
+            put(st.toff[q.x >> cshift] + r0, x, y, z, w);
+            put(st.toff[q.y >> cshift] + r1, y, x, z, w);
+            put(st.toff[q.z >> cshift] + r2, z, x, y, w);
+            put(st.toff[q.w >> cshift] + r3, w, x, y, z);
+        }
+        __syncthreads();
+        stage_flush(st, nc, out, L1_THREADS);
+        __syncthreads();
+        for (int i = tid; i < nc; i += L1_THREADS) st.gbase[i] += st.cnt[i];
+    }
+}
+
+// ---- L2: slices of a coarse bin -> buckets ---------------------------------
+constexpr int L2_THREADS = 1024;
+
+__device__ __forceinline__ void slice_range(const int32_t *h1s, int32_t nch, int32_t c,
+                                            int32_t s, int32_t S, int32_t &lo, int32_t &hi) {
+    const int64_t rb = h1s[(int64_t)c * nch], re = h1s[(int64_t)(c + 1) * nch];
+    lo = (int32_t)(rb + (re - rb) * s / S);
+    hi = (int32_t)(rb + (re - rb) * (s + 1) / S);
+}
+
+__global__ __launch_bounds__(L2_THREADS) void k_l2_count(const uint32_t *__restrict__ w0,
+                                                          const int32_t *__restrict__ h1s,
+                                                          int32_t nch, int bv_log, int fs,
+                                                          int32_t nb, int32_t *__restrict__ h2) {
+    __shared__ int32_t h[MAX_FINE];
+    const int32_t c = blockIdx.x, s = blockIdx.y, S = gridDim.y;
+    const int F = 1 << fs;
+    for (int i = threadIdx.x; i < F; i += L2_THREADS) h[i] = 0;
+    __syncthreads();
+    int32_t lo, hi;
+    slice_range(h1s, nch, c, s, S, lo, hi);
+    const int32_t fb0 = c << fs;
+    for (int32_t i = lo + threadIdx.x; i < hi; i += L2_THREADS)
+        atomicAdd(&h[(int32_t)((w0[i] & M24) >> bv_log) - fb0], 1);
+    __syncthreads();
+    for (int i = threadIdx.x; i < F && fb0 + i < nb; i += L2_THREADS)
+        h2[(int64_t)(fb0 + i) * S + s] = h[i];
+}
+
+__global__ __launch_bounds__(L2_THREADS) void k_l2_scatter(const Rec in,
+                                                            const int32_t *__restrict__ h1s,
+                                                            int32_t nch, int bv_log, int fs,
+                                                            int32_t nb,
+                                                            const int32_t *__restrict__ h2s,
+                                                            Rec out) {
+    __shared__ Stage st;
+    const int tid = threadIdx.x;
+    const int32_t c = blockIdx.x, s = blockIdx.y, S = gridDim.y;
+    const int32_t fb0 = c << fs;
+    const int F = min(1 << fs, nb - fb0);
+    for (int i = tid; i < F; i += L2_THREADS) st.gbase[i] = h2s[(int64_t)(fb0 + i) * S + s];
+    int32_t lo, hi;
+    slice_range(h1s, nch, c, s, S, lo, hi);
+    constexpr int PER = STAGE / L2_THREADS;
+    for (int32_t ib = lo; ib < hi; ib += STAGE) {
+        for (int i = tid; i < F; i += L2_THREADS) st.cnt[i] = 0;
+        __syncthreads();
+        uint32_t a[PER], b[PER], d[PER];
+        int f[PER], r[PER];
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int32_t i = ib + k * L2_THREADS + tid;
+            f[k] = -1;
+            if (i < hi) {
+                a[k] = in.w0[i];
+                b[k] = in.w1[i];
+                d[k] = in.w2[i];
+                f[k] = (int)((a[k] & M24) >> bv_log) - fb0;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < PER; k++)
+            if (f[k] >= 0) r[k] = atomicAdd(&st.cnt[f[k]], 1);
+        __syncthreads();
+        stage_scan(st, F);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < PER; k++)
+            if (f[k] >= 0) {
+                const int slot = st.toff[f[k]] + r[k];
+                st.w0[slot] = a[k];
+                st.w1[slot] = b[k];
+                st.w2[slot] = d[k];
+            }
+        __syncthreads();
+        stage_flush(st, F, out, L2_THREADS);
+        __syncthreads();
+        for (int i = tid; i < F; i += L2_THREADS) st.gbase[i] += st.cnt[i];
+    }
+}
+
+// ---- B: per-bucket neighbour sets, link components, sorted rows --------------
+__device__ __forceinline__ int hash_insert(int32_t *tab, uint32_t *occ, int32_t u) {
+    const uint32_t h = ((uint32_t)u * 2654435769u) >> 27;
+#pragma unroll 1
+    for (int p = 0; p < SLOTS; p++) {
+        const int s = (int)((h + (uint32_t)p) & (SLOTS - 1));
+        const int32_t cur = tab[s];
+        if (cur == u) return s;
+        if (cur == -1) {
+            const int32_t old = atomicCAS(&tab[s], -1, u);
+            if (old == -1) {
+                atomicOr(occ, 1u << s);
+                return s;
+            }
+            if (old == u) return s;
+        }
+    }
+    return -1;
+}
+
+__device__ __forceinline__ uint32_t n_components(const uint32_t *adj, uint32_t mask) {
+    uint32_t cnt = 0, rem = mask;
+    while (rem) {
+        uint32_t comp = rem & (0u - rem);
+        uint32_t front = comp;
+        while (front) {
+            const int f = __ffs(front) - 1;
+            front &= front - 1u;
+            const uint32_t nb = adj[f] & mask & ~comp;
+            comp |= nb;
+            front |= nb;
+        }
+        rem &= ~comp;
+        cnt++;
+    }
+    return cnt;
+}
+
+// Bitonic sorting network over 32 registers (ascending).
+__device__ __forceinline__ void sort32(int32_t (&r)[32]) {
+#pragma unroll
+    for (int k = 2; k <= 32; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+            for (int i = 0; i < 32; i++) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const int32_t a = r[i], b = r[l];
+                    const bool up = (i & k) == 0;
+                    r[i] = up ? min(a, b) : max(a, b);
+                    r[l] = up ? max(a, b) : min(a, b);
+                }
+            }
+}
+
+template <int BV_LOG, typename S>
+__global__ __launch_bounds__(2 << BV_LOG) void k_bkt_vv_cp(
+    const Rec rec, const int32_t *__restrict__ h2s, int32_t nslice,
+    const S *__restrict__ scal, int32_t V, int classify, int32_t *__restrict__ scratch,
+    int32_t *__restrict__ deg, int32_t *__restrict__ rowsrc, uint32_t *__restrict__ classes,
+    int32_t *__restrict__ heavy, int32_t *__restrict__ fix, int32_t *__restrict__ counters) {
+    constexpr int BV = 1 << BV_LOG;
+    constexpr int NT = 2 * BV;
+    constexpr int BATCH = 4;
+    constexpr int PER_T = BV * SLOTS / NT;  // slots per thread in phase 2 (16)
+    __shared__ int32_t key[BV * SLOTS];
+    __shared__ uint32_t adj[BV * SLOTS];
+    __shared__ uint32_t occ[BV], lowm[BV];
+    __shared__ int32_t ccount[BV];
+    __shared__ int32_t ovf[BV];
+    __shared__ S sv[BV];
+    __shared__ int32_t wsum[2][NT / 64];
+
+    const int tid = threadIdx.x;
+    const int32_t b = blockIdx.x;
+    const int32_t v0 = b << BV_LOG;
+    const int nv = min(BV, V - v0);
+    const int32_t cbeg = h2s[(int64_t)b * nslice], cend = h2s[(int64_t)(b + 1) * nslice];
+
+    for (int i = tid; i < BV * SLOTS; i += NT) {
+        key[i] = -1;
+        adj[i] = 0u;
+    }
+    if (tid < BV) {
+        occ[tid] = 0u;
+        lowm[tid] = 0u;
+        ccount[tid] = 0;
+        ovf[tid] = 0;
+        sv[tid] = (classify && tid < nv) ? scal[v0 + tid] : S(0);
+    }
+    __syncthreads();
+
+    // phase 1: neighbour sets and shared-tet link adjacency (loads batched)
+    for (int32_t i0 = cbeg + tid; i0 < cend; i0 += BATCH * NT) {
+        uint32_t w[BATCH][3];
+#pragma unroll
+        for (int k = 0; k < BATCH; k++) {
+            const int32_t i = i0 + k * NT;
+            if (i < cend) {
+                w[k][0] = rec.w0[i];
+                w[k][1] = rec.w1[i];
+                w[k][2] = rec.w2[i];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < BATCH; k++) {
+            if (i0 + k * NT >= cend) break;
+            uint32_t v, x, y, z;
+            rec_unpack(w[k][0], w[k][1], w[k][2], v, x, y, z);
+            const int vl = (int)(v & (BV - 1));
+            int32_t *tab = key + vl * SLOTS;
+            atomicAdd(&ccount[vl], 1);
+            const int sa = hash_insert(tab, &occ[vl], (int32_t)x);
+            const int sb = hash_insert(tab, &occ[vl], (int32_t)y);
+            const int sc = hash_insert(tab, &occ[vl], (int32_t)z);
+            if ((sa | sb | sc) < 0) {
+                ovf[vl] = 1;
+            } else {
+                uint32_t *av = adj + vl * SLOTS;
+                atomicOr(&av[sa], (1u << sb) | (1u << sc));
+                atomicOr(&av[sb], (1u << sa) | (1u << sc));
+                atomicOr(&av[sc], (1u << sa) | (1u << sb));
+            }
+        }
+    }
+    __syncthreads();
+
+    // phase 2: lower/upper mask of every slot; a half-wave covers one vertex's
+    // 32 slots, so the mask is a ballot
+    if (classify) {
+        int32_t u[PER_T];
+        S su[PER_T];
+#pragma unroll
+        for (int k = 0; k < PER_T; k++) {
+            const int idx = tid + k * NT;
+            u[k] = (idx / SLOTS < nv) ? key[idx] : -1;
+        }
+#pragma unroll
+        for (int k = 0; k < PER_T; k++) su[k] = u[k] >= 0 ? scal[u[k]] : S(0);
+#pragma unroll
+        for (int k = 0; k < PER_T; k++) {
+            const int idx = tid + k * NT;
+            const int vl = idx / SLOTS;
+            const bool lo = u[k] >= 0 && is_lower(su[k], sv[vl], u[k], v0 + vl);
+            const uint64_t bal = __ballot(lo);
+            if ((tid & 31) == 0)
+                lowm[vl] = (uint32_t)(bal >> (tid & 32));
+        }
+    }
+    __syncthreads();
+
+    // phase 3: per vertex (lanes < nv): components, row registers
+    const int vl = tid;
+    const bool act = vl < nv;
+    const bool hv = act && ovf[vl] != 0;
+    const bool ok = act && !hv;
+    int32_t r[32];
+    uint32_t nl = 0, nu = 0;
+    int d = 0;
+    if (ok) {
+        const uint32_t om = occ[vl];
+        d = __popc(om);
+        if (classify) {
+            const uint32_t lm = lowm[vl];
+            nl = n_components(adj + vl * SLOTS, om & lm);
+            nu = n_components(adj + vl * SLOTS, om & ~lm);
+        }
+#pragma unroll
+        for (int i = 0; i < 32; i++) {
+            const int32_t k = key[vl * SLOTS + i];
+            r[i] = k < 0 ? INT32_MAX : k;
+        }
+    }
+    const int32_t hres = hv ? 3 * ccount[vl] : 0;
+    __syncthreads();  // every lane has read its keys; key[] becomes row staging
+
+    // block exclusive scan of (row lengths, heavy reservations)
+    const int32_t x = ok ? d : 0, y = hres;
+    const int32_t xi = tgd::wave_inclusive_sum(x), yi = tgd::wave_inclusive_sum(y);
+    const int lane = __lane_id(), wid = tid / 64;
+    if (lane == 63) {
+        wsum[0][wid] = xi;
+        wsum[1][wid] = yi;
+    }
+    __syncthreads();
+    int32_t xb = 0, yb = 0, xt = 0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; w++) {
+        if (w < wid) {
+            xb += wsum[0][w];
+            yb += wsum[1][w];
+        }
+        xt += wsum[0][w];
+    }
+    const int32_t loff = hv ? xt + yb + yi - y : xb + xi - x;
+    if (ok) {
+        sort32(r);
+#pragma unroll
+        for (int i = 0; i < 32; i++)
+            if (i < d) key[loff + i] = r[i];
+        const int32_t v = v0 + vl;
+        deg[v] = d;
+        if (classify) {
+            classes[3 * (int64_t)v] = nl;
+            classes[3 * (int64_t)v + 1] = nu;
+            classes[3 * (int64_t)v + 2] = verdict(nl, nu);
+        }
+    }
+    if (act) rowsrc[v0 + vl] = 3 * cbeg + loff;
+    wave_append(heavy, counters + C_HEAVY, v0 + vl, hv);
+    wave_append(fix, counters + C_FIX, v0 + vl, ok && classify && (nl >= 2 || nu >= 2));
+    int dm = ok ? d : 0;
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) dm = max(dm, __shfl_xor(dm, s, 64));
+    if (lane == 0 && dm > 0) atomicMax(counters + C_MAXDEG, dm);
+    __syncthreads();
+
+    // phase 4: coalesced store of the bucket's rows
+    int32_t *dst = scratch + 3 * (int64_t)cbeg;
+    for (int i = tid; i < xt; i += NT) dst[i] = key[i];
+}
+
+// Serial fallback for vertices with more than 32 neighbours: lane 0 scans
+// its bucket's corner records, keeping a 4096-entry neighbour list in LDS.
+template <int BV_LOG, typename S>
+__global__ __launch_bounds__(64) void k_bkt_heavy(
+    const Rec rec, const int32_t *__restrict__ h2s, int32_t nslice,
+    const S *__restrict__ scal, int classify, int32_t *__restrict__ scratch,
+    int32_t *__restrict__ deg, const int32_t *__restrict__ rowsrc,
+    uint32_t *__restrict__ classes, const int32_t *__restrict__ heavy,
+    int32_t *__restrict__ fix, int32_t *__restrict__ fixbig, int32_t *__restrict__ counters) {
+    __shared__ int32_t list[HEAVY_CAP];
+    __shared__ uint16_t par[HEAVY_CAP];
+    __shared__ uint8_t lowf[HEAVY_CAP];
+    if (threadIdx.x != 0) return;
+    const int32_t n = counters[C_HEAVY];
+    for (int32_t h = blockIdx.x; h < n; h += gridDim.x) {
+        const int32_t v = heavy[h];
+        const int32_t bkt = v >> BV_LOG;
+        const int32_t cbeg = h2s[(int64_t)bkt * nslice], cend = h2s[(int64_t)(bkt + 1) * nslice];
+        const S sv = classify ? scal[v] : S(0);
+        int m = 0;
+        bool over = false;
+        auto find_or_insert = [&](int32_t u) -> int {
+            for (int i = 0; i < m; i++)
+                if (list[i] == u) return i;
+            if (m == HEAVY_CAP) return -1;
+            list[m] = u;
+            par[m] = (uint16_t)m;
+            lowf[m] = classify ? (uint8_t)is_lower(scal[u], sv, u, v) : 0;
+            return m++;
+        };
+        auto root = [&](int x) {
+            while (par[x] != x) {
+                par[x] = par[par[x]];
+                x = par[x];
+            }
+            return x;
+        };
+        auto unite = [&](int a, int b) {
+            if (lowf[a] != lowf[b]) return;
+            a = root(a);
+            b = root(b);
+            if (a == b) return;
+            if (a < b) par[b] = (uint16_t)a;
+            else par[a] = (uint16_t)b;
+        };
+        for (int32_t i = cbeg; i < cend && !over; i++) {
+            uint32_t rv, x, y, z;
+            rec_unpack(rec.w0[i], rec.w1[i], rec.w2[i], rv, x, y, z);
+            if ((int32_t)rv != v) continue;
+            const int i0 = find_or_insert((int32_t)x);
+            const int i1 = find_or_insert((int32_t)y);
+            const int i2 = find_or_insert((int32_t)z);
+            if ((i0 | i1 | i2) < 0) {
+                over = true;
+                break;
+            }
+            if (classify) {
+                unite(i0, i1);
+                unite(i0, i2);
+                unite(i1, i2);
+            }
+        }
+        if (over) {
+            atomicOr(counters + C_ERR, tg::ERRBIT_CAPACITY);
+            deg[v] = 0;
+            if (classify) {
+                classes[3 * (int64_t)v] = 0;
+                classes[3 * (int64_t)v + 1] = 0;
+                classes[3 * (int64_t)v + 2] = 0;
+            }
+            continue;
+        }
+        uint32_t nl = 0, nu = 0;
+        if (classify) {
+            for (int i = 0; i < m; i++)
+                if (par[i] == i) {
+                    if (lowf[i]) nl++;
+                    else nu++;
+                }
+            classes[3 * (int64_t)v] = nl;
+            classes[3 * (int64_t)v + 1] = nu;
+            classes[3 * (int64_t)v + 2] = verdict(nl, nu);
+        }
+        deg[v] = m;
+        atomicMax(counters + C_MAXDEG, m);
+        for (int i = 1; i < m; i++) {
+            const int32_t x = list[i];
+            int j = i - 1;
+            while (j >= 0 && list[j] > x) {
+                list[j + 1] = list[j];
+                j--;
+            }
+            list[j + 1] = x;
+        }
+        int32_t *dst = scratch + rowsrc[v];
+        for (int i = 0; i < m; i++) dst[i] = list[i];
+        if (classify && (nl >= 2 || nu >= 2)) {
+            if (m <= 64) fix[atomicAdd(counters + C_FIX, 1)] = v;
+            else fixbig[atomicAdd(counters + C_FIXBIG, 1)] = v;
+        }
+    }
+}
+
+template <typename S>
+int launch_bucketed_typed(tg::BucketBuffers &k, bool classify, hipStream_t s) {
+    const tg::BucketPlan &p = k.plan;
+    tg::VvCpBuffers &b = k.base;
+    const int32_t V = (int32_t)b.n_points;
+    const int64_t T = b.n_cells;
+    const int cls = classify ? 1 : 0;
+    const S *scal = (const S *)b.scalars;
+    const int4 *tv4 = (const int4 *)b.tv;
+    tg::StageMarks *mk = b.marks;
+    const int64_t n = 4 * T;
+    const Rec r1{k.rec1, k.rec1 + n, k.rec1 + 2 * n};
+    const Rec r2{k.rec2, k.rec2 + n, k.rec2 + 2 * n};
+    const int cshift = p.bv_log + p.fs;
+
+    hipLaunchKernelGGL(k_l1_count, dim3(p.nchunks), dim3(L1_THREADS), 0, s, tv4, T, p.tpc,
+                       cshift, p.nc, p.nchunks, k.h1);
+    TG_LAUNCH_CHECK("k_l1_count");
+    int rc = tg::scan_exclusive_i32(k.h1, k.h1, (int64_t)p.nc * p.nchunks, b.scan_tmp, s);
+    if (rc) return rc;
+    if (mk) mk->mark("l1_count", s);
+    hipLaunchKernelGGL(k_l1_scatter, dim3(p.nchunks), dim3(L1_THREADS), 0, s, tv4, T, p.tpc,
+                       cshift, p.nc, p.nchunks, k.h1, r1);
+    TG_LAUNCH_CHECK("k_l1_scatter");
+    if (mk) mk->mark("l1_scatter", s);
+    const dim3 g2(p.nc, p.slices);
+    hipLaunchKernelGGL(k_l2_count, g2, dim3(L2_THREADS), 0, s, r1.w0, k.h1, p.nchunks, p.bv_log,
+                       p.fs, p.nb, k.h2);
+    TG_LAUNCH_CHECK("k_l2_count");
+    if ((rc = tg::scan_exclusive_i32(k.h2, k.h2, (int64_t)p.nb * p.slices, b.scan_tmp, s)))
+        return rc;
+    if (mk) mk->mark("l2_count", s);
+    hipLaunchKernelGGL(k_l2_scatter, g2, dim3(L2_THREADS), 0, s, r1, k.h1, p.nchunks, p.bv_log,
+                       p.fs, p.nb, k.h2, r2);
+    TG_LAUNCH_CHECK("k_l2_scatter");
+    if (mk) mk->mark("l2_scatter", s);
+    if (p.bv_log == 7)
+        hipLaunchKernelGGL((k_bkt_vv_cp<7, S>), dim3(p.nb), dim3(256), 0, s, r2, k.h2, p.slices,
+                           scal, V, cls, b.scratch, b.deg, k.rowsrc, b.classes, b.heavy, b.fix,
+                           b.counters);
+    else
+        hipLaunchKernelGGL((k_bkt_vv_cp<8, S>), dim3(p.nb), dim3(512), 0, s, r2, k.h2, p.slices,
+                           scal, V, cls, b.scratch, b.deg, k.rowsrc, b.classes, b.heavy, b.fix,
+                           b.counters);
+    TG_LAUNCH_CHECK("k_bkt_vv_cp");
+    if (mk) mk->mark("bkt_vv_cp", s);
+    if (p.bv_log == 7)
+        hipLaunchKernelGGL((k_bkt_heavy<7, S>), dim3(256), dim3(64), 0, s, r2, k.h2, p.slices,
+                           scal, cls, b.scratch, b.deg, k.rowsrc, b.classes, b.heavy, b.fix,
+                           b.fixbig, b.counters);
+    else
+        hipLaunchKernelGGL((k_bkt_heavy<8, S>), dim3(256), dim3(64), 0, s, r2, k.h2, p.slices,
+                           scal, cls, b.scratch, b.deg, k.rowsrc, b.classes, b.heavy, b.fix,
+                           b.fixbig, b.counters);
+    TG_LAUNCH_CHECK("k_bkt_heavy");
+    if (mk) mk->mark("bkt_heavy", s);
+    return tg::OK;
+}
+
+int ceil_log2(int64_t x) {
+    int l = 0;
+    while (((int64_t)1 << l) < x) l++;
+    return l;
+}
+
+}  // namespace
+
+namespace tg {
+
+BucketPlan plan_buckets(int64_t n_cells, int64_t n_points, int bv_log) {
+    BucketPlan p;
+    const int64_t V = n_points, T = n_cells;
+    p.bv_log = bv_log;
+    const int64_t nb = std::max<int64_t>(1, (V + (1 << bv_log) - 1) >> bv_log);
+    p.nb = (int32_t)nb;
+    p.fs = std::max(0, ceil_log2(nb) - 8);  // coarse bins <= 256
+    p.nc = (int32_t)((nb + (1 << p.fs) - 1) >> p.fs);
+    p.ok = V > 0 && V <= (int64_t)M24 + 1 && (1 << p.fs) <= MAX_FINE && p.nc <= MAX_COARSE;
+    p.tpc = 8192;
+    p.nchunks = (int32_t)std::max<int64_t>(1, (T + p.tpc - 1) / p.tpc);
+    p.slices = std::max(1, std::min(64, (1024 + p.nc - 1) / p.nc));
+    return p;
+}
+
+int launch_bucketed_vv_cp(BucketBuffers &k, bool classify, hipStream_t s) {
+    return k.base.scalars_f64 ? launch_bucketed_typed<double>(k, classify, s)
+                              : launch_bucketed_typed<float>(k, classify, s);
+}
+
+}  // namespace tg

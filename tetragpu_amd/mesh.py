@@ -110,8 +110,8 @@ class Mesh:
         self._sc_keep = sc
         check(self._lib.tg_mesh_set_scalars(self._h, sc.ctypes.data, int(f64), 0))
 
-    def set_config(self, lane_cap: int, lane_block: int) -> None:
-        check(self._lib.tg_mesh_set_config(self._h, lane_cap, lane_block))
+    def set_config(self, lane_cap: int = 32, lane_block: int = 128, variant: int = 0) -> None:
+        check(self._lib.tg_mesh_set_config(self._h, variant, lane_cap, lane_block))
 
     def build(self, mask: int) -> None:
         check(self._lib.tg_build(self._h, mask))
@@ -130,6 +130,13 @@ class Mesh:
         v, t, e, f = (C.c_int64() for _ in range(4))
         check(self._lib.tg_mesh_counts(self._h, C.byref(v), C.byref(t), C.byref(e), C.byref(f)))
         return {"V": v.value, "T": t.value, "E": e.value, "F": f.value}
+
+    def shape(self, name: str) -> dict:
+        rows, nnz, width, fixed = C.c_int64(), C.c_int64(), C.c_int32(), C.c_int32()
+        check(self._lib.tg_relation_shape(self._h, REL[name], C.byref(rows), C.byref(nnz),
+                                          C.byref(width), C.byref(fixed)))
+        return {"rows": rows.value, "nnz": nnz.value, "width": width.value,
+                "fixed": bool(fixed.value)}
 
     def relation(self, name: str):
         """(offsets int64 [rows+1] or None, values int32 [nnz, width])."""
@@ -159,6 +166,18 @@ class Mesh:
         out = np.empty(self.n_points, dtype=np.uint8)
         check(self._lib.tg_labels6_copy(self._h, out.ctypes.data))
         return out
+
+    def profile(self, on: bool = True) -> None:
+        check(self._lib.tg_profile_enable(self._h, int(on)))
+
+    def stage_times(self) -> list[tuple[str, float]]:
+        n = C.c_int32()
+        ms = (C.c_float * 32)()
+        names = C.create_string_buffer(32 * 32)
+        check(self._lib.tg_profile_read(self._h, 32, C.byref(n), ms, names))
+        raw = names.raw
+        return [(raw[32 * i:32 * i + 32].split(b"\0")[0].decode(), float(ms[i]))
+                for i in range(min(n.value, 32))]
 
     def approx_max_vv(self) -> int:
         r = C.c_int32()
