@@ -162,3 +162,32 @@ def test_bucket_boundaries_partial_last_bucket(gpu):
     m = gpu.kuhn(7, seed=23)
     check(gpu, m.cells, m.n_points + 77, np.concatenate([m.scalars, np.zeros(77, np.float32)]),
           variant=2)
+
+
+def soup(seed, V, T):
+    """Random tets (4 distinct vertices each) over V vertices: non-manifold,
+    full of empty triangles, so the row-membership link rule merges
+    components the shared-tet rule keeps apart."""
+    rng = np.random.default_rng(seed)
+    cells = np.array([rng.choice(V, 4, replace=False) for _ in range(T)], np.int32)
+    sc = rng.integers(0, 8, V).astype(np.float32)  # many ties
+    return cells, sc
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("seed,V,T", [(1, 30, 40), (2, 200, 300), (3, 60, 400),
+                                      (4, 40, 1500), (5, 400, 700)])
+def test_random_soup(gpu, variant, seed, V, T):
+    cells, sc = soup(seed, V, T)
+    check(gpu, cells, V, sc, variant=variant)
+
+
+def test_non_flag_link_merges_components(gpu):
+    # v=0 sits in two tets that share only v; tet (1, 4, 7, 8) joins neighbour
+    # 1 of the first with neighbour 4 of the second without containing v
+    cells = np.array([[0, 1, 2, 3], [0, 4, 5, 6], [1, 4, 7, 8]], np.int32)
+    sc = np.array([10, 1, 2, 3, 4, 5, 6, 20, 20], np.float32)
+    for variant in VARIANTS:
+        cls = check(gpu, cells, 9, sc, variant=variant)
+        # one lower component after the merge: a maximum, not a saddle
+        assert cls[0].tolist() == [1, 0, 2]

@@ -1,19 +1,928 @@
-// Placeholder: id-assigning relations are not built yet.
+// Edge and face ids in the reference's first-occurrence order, and the
+// relations derived from them (include/cuda_extraction.h relation set).
+//
+// Edges (src/cpu_extraction.cpp:28-92): the edge {lo, hi} gets the rank of
+// its first position 6c + k (cells ascending, pair k over (0,1) (0,2) (0,3)
+// (1,2) (1,3) (2,3)) among all edges' first positions.  One lane per vertex
+// v walks v's tets in ascending order and records, for every upper
+// neighbour u > v (an entry of the ascending VV row), the first position it
+// meets; first positions are flagged in a byte array, packed to a bitmap
+// whose word popcounts are scanned, and an edge's id is the rank of its
+// first position.  No atomics.
+//
+// Faces (src/cpu_extraction.cpp:122-222): slot s of cell c is the positional
+// triple (cv[lo], cv[mid], cv[hi]) with (lo, mid, hi) = (0,1,2) (0,1,3)
+// (0,2,3) (1,2,3), position 4c + s; the key is not sorted.  The cells holding
+// all three vertices are ET(lo,mid) n ET(lo,hi); the first occurrence of a
+// key is the smallest position among those cells' slots with the same
+// positional triple, and face ids are ranks of first occurrences.
+//
+// The remaining relations are two-pass CSR builds (count, scan, fill) over
+// sorted rows: ET(e) = VT(lo) n VT(hi); EF, TT, FF, EE by merging sorted
+// lists; FT from the face matches.
+#include <algorithm>
+
 #include "ids.h"
+#include "tetra_c.h"
+
+namespace {
+
+constexpr int INF = INT32_MAX;
+
+// pair index of corner positions i < j: (0,1)0 (0,2)1 (0,3)2 (1,2)3 (1,3)4 (2,3)5
+__device__ __forceinline__ int pair_index(int i, int j) {
+    return i == 0 ? j - 1 : (i == 1 ? j + 1 : 5);
+}
+
+__device__ __forceinline__ int32_t corner(const int4 &q, int j) {
+    return j == 0 ? q.x : j == 1 ? q.y : j == 2 ? q.z : q.w;
+}
+
+// positions (lo, mid, hi) of face slot s
+__device__ __forceinline__ void face_pos(int s, int &lo, int &mid, int &hi) {
+    lo = s == 3 ? 1 : 0;
+    mid = s <= 1 ? 1 : 2;
+    hi = s == 0 ? 2 : 3;
+}
+
+// index of x in the ascending array a[0..n), or -1
+__device__ __forceinline__ int32_t bsearch_i32(const int32_t *a, int32_t n, int32_t x) {
+    int32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (a[mid] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < n && a[lo] == x) ? lo : -1;
+}
+
+// ---- edges --------------------------------------------------------------------
+// Lane per vertex v: first position of every upper edge (v, u>v), flag it.
+__global__ void k_edge_first(const int4 *__restrict__ tv4, const int32_t *__restrict__ vt_off,
+                             const int32_t *__restrict__ vtc, const int32_t *__restrict__ vv_off,
+                             const int32_t *__restrict__ vv, int64_t V,
+                             int32_t *__restrict__ efirst, uint8_t *__restrict__ flags) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t rb = vv_off[v], rn = vv_off[v + 1] - rb;
+        const int32_t *row = vv + rb;
+        int32_t *ef = efirst + rb;
+        for (int32_t k = vt_off[v]; k < vt_off[v + 1]; k++) {
+            const int32_t p = vtc[k];
+            const int32_t t = p >> 2, jv = p & 3;
+            const int4 q = tv4[t];
+            for (int j = 0; j < 4; j++) {
+                if (j == jv) continue;
+                const int32_t u = corner(q, j);
+                if (u < v) continue;
+                const int32_t idx = bsearch_i32(row, rn, u);
+                if (idx >= 0 && ef[idx] == INF)
+                    ef[idx] = 6 * t + pair_index(min(j, jv), max(j, jv));
+            }
+        }
+        for (int32_t i = 0; i < rn; i++)
+            if (row[i] > v && ef[i] != INF) flags[ef[i]] = 1;
+    }
+}
+
+// bytes -> bitmap words and their popcounts
+__global__ void k_pack_flags(const uint8_t *__restrict__ flags, int64_t n,
+                             uint32_t *__restrict__ bits, int32_t *__restrict__ pop) {
+    const int64_t nw = (n + 31) / 32;
+    for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nw;
+         w += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t m = 0;
+        const int64_t b = 32 * w;
+        for (int i = 0; i < 32 && b + i < n; i++) m |= (uint32_t)(flags[b + i] != 0) << i;
+        bits[w] = m;
+        pop[w] = __popc(m);
+    }
+}
+
+__device__ __forceinline__ int32_t rank_of(const uint32_t *bits, const int32_t *wpre, int32_t p) {
+    const int32_t w = p >> 5;
+    return wpre[w] + __popc(bits[w] & ((1u << (p & 31)) - 1u));
+}
+
+// Lane per vertex v: ids of v's upper edges (VV-aligned) and EV.
+__global__ void k_edge_ids(const int32_t *__restrict__ vv_off, const int32_t *__restrict__ vv,
+                           int64_t V, const int32_t *__restrict__ efirst,
+                           const uint32_t *__restrict__ bits, const int32_t *__restrict__ wpre,
+                           int32_t *__restrict__ vv_eid, int32_t *__restrict__ ev) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        for (int32_t i = vv_off[v]; i < vv_off[v + 1]; i++) {
+            const int32_t u = vv[i];
+            if (u < v || efirst[i] == INF) continue;
+            const int32_t id = rank_of(bits, wpre, efirst[i]);
+            vv_eid[i] = id;
+            ev[2 * (int64_t)id] = (int32_t)v;
+            ev[2 * (int64_t)id + 1] = u;
+        }
+    }
+}
+
+// VE rows {hi, id}: lane per vertex v walks its tets ascending; a pair whose
+// low vertex is v and whose position is a first occurrence mints the next id
+// of v's row, so rows come out in ascending id order.
+template <bool FILL>
+__global__ void k_ve(const int4 *__restrict__ tv4, const int32_t *__restrict__ vt_off,
+                     const int32_t *__restrict__ vtc, int64_t V, const uint8_t *__restrict__ flags,
+                     const uint32_t *__restrict__ bits, const int32_t *__restrict__ wpre,
+                     int32_t *__restrict__ cnt_or_off, int32_t *__restrict__ out) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        int32_t n = 0;
+        int32_t *row = FILL ? out + 2 * (int64_t)cnt_or_off[v] : nullptr;
+        for (int32_t k = vt_off[v]; k < vt_off[v + 1]; k++) {
+            const int32_t p = vtc[k];
+            const int32_t t = p >> 2, jv = p & 3;
+            const int4 q = tv4[t];
+            for (int i = 0; i < 3; i++)
+                for (int j = i + 1; j < 4; j++) {
+                    if (i != jv && j != jv) continue;
+                    const int32_t u = corner(q, i == jv ? j : i);
+                    if (u < v) continue;
+                    const int32_t pos = 6 * t + pair_index(i, j);
+                    if (!flags[pos]) continue;
+                    if (FILL) {
+                        row[2 * n] = u;
+                        row[2 * n + 1] = rank_of(bits, wpre, pos);
+                    }
+                    n++;
+                }
+        }
+        if (!FILL) cnt_or_off[v] = n;
+    }
+}
+
+// lower entries (u < v) take the id stored in u's row
+__global__ void k_edge_lower(const int32_t *__restrict__ vv_off, const int32_t *__restrict__ vv,
+                             int64_t V, int32_t *__restrict__ vv_eid) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        for (int32_t i = vv_off[v]; i < vv_off[v + 1]; i++) {
+            const int32_t u = vv[i];
+            if (u > v) continue;
+            const int32_t ub = vv_off[u];
+            const int32_t j = bsearch_i32(vv + ub, vv_off[u + 1] - ub, (int32_t)v);
+            vv_eid[i] = j >= 0 ? vv_eid[ub + j] : -1;
+        }
+    }
+}
+
+__device__ __forceinline__ int32_t edge_id(const int32_t *vv_off, const int32_t *vv,
+                                           const int32_t *vv_eid, int32_t a, int32_t b) {
+    const int32_t lo = min(a, b), hi = max(a, b);
+    const int32_t rb = vv_off[lo];
+    const int32_t j = bsearch_i32(vv + rb, vv_off[lo + 1] - rb, hi);
+    return j >= 0 ? vv_eid[rb + j] : -1;
+}
+
+__global__ void k_te(const int4 *__restrict__ tv4, int64_t T, const int32_t *__restrict__ vv_off,
+                     const int32_t *__restrict__ vv, const int32_t *__restrict__ vv_eid,
+                     int32_t *__restrict__ te) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < T;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int4 q = tv4[t];
+        int k = 0;
+        for (int i = 0; i < 3; i++)
+            for (int j = i + 1; j < 4; j++, k++)
+                te[6 * t + k] = edge_id(vv_off, vv, vv_eid, corner(q, i), corner(q, j));
+    }
+}
+
+// ---- ET = VT(lo) n VT(hi) -------------------------------------------------------
+template <bool FILL>
+__global__ void k_et(const int32_t *__restrict__ ev, int64_t E, int64_t V,
+                     const int32_t *__restrict__ vt_off, const int32_t *__restrict__ vtc,
+                     int64_t *__restrict__ cnt_or_off, int32_t *__restrict__ out) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t a = ev[2 * e], b = ev[2 * e + 1];
+        const bool ok = a >= 0 && a < V && b >= 0 && b < V;  // always, for a built EV
+        int32_t i = ok ? vt_off[a] : 0, ie = ok ? vt_off[a + 1] : 0;
+        int32_t j = ok ? vt_off[b] : 0, je = ok ? vt_off[b + 1] : 0;
+        int64_t n = 0, o = FILL ? cnt_or_off[e] : 0;
+        while (i < ie && j < je) {
+            const int32_t x = vtc[i] >> 2, y = vtc[j] >> 2;
+            if (x < y) i++;
+            else if (y < x) j++;
+            else {
+                if (FILL) out[o + n] = x;
+                n++;
+                i++;
+                j++;
+            }
+        }
+        if (!FILL) cnt_or_off[e] = n;
+    }
+}
+
+// ---- faces ----------------------------------------------------------------------
+// tets holding face slot s of tet t (ascending) via ET(lo,mid) n ET(lo,hi);
+// returns the first matching position (smallest 4t'+s' with the same
+// positional triple) and counts all matching tets.
+// Ids read from TE and ET are checked against E and T: an inconsistent
+// relation yields INF (reported by the callers) instead of a stray access.
+__device__ __forceinline__ int32_t face_first(const int4 *tv4, int64_t T, const int32_t *te,
+                                              int64_t E, const int64_t *et_off,
+                                              const int32_t *et, int32_t t, int s,
+                                              int32_t &nmatch, int32_t *tets2) {
+    const int4 q = tv4[t];
+    int lo, mid, hi;
+    face_pos(s, lo, mid, hi);
+    const int32_t a = corner(q, lo), b = corner(q, mid), c = corner(q, hi);
+    const int32_t e1 = te[6 * (int64_t)t + pair_index(lo, mid)];
+    const int32_t e2 = te[6 * (int64_t)t + pair_index(lo, hi)];
+    int32_t best = INF;
+    nmatch = 0;
+    if (e1 < 0 || e1 >= E || e2 < 0 || e2 >= E) return best;
+    int64_t i = et_off[e1], ie = et_off[e1 + 1], j = et_off[e2], je = et_off[e2 + 1];
+    while (i < ie && j < je) {
+        const int32_t x = et[i], y = et[j];
+        if (x < y) i++;
+        else if (y < x) j++;
+        else if (x < 0 || x >= T) return INF;
+        else {
+            const int4 r = tv4[x];
+            for (int s2 = 0; s2 < 4; s2++) {
+                int l2, m2, h2;
+                face_pos(s2, l2, m2, h2);
+                if (corner(r, l2) == a && corner(r, m2) == b && corner(r, h2) == c) {
+                    if (nmatch < 2 && tets2) tets2[nmatch] = x;
+                    nmatch++;
+                    best = min(best, 4 * x + s2);
+                    break;
+                }
+            }
+            i++;
+            j++;
+        }
+    }
+    return best;
+}
+
+__global__ void k_face_first(const int4 *__restrict__ tv4, int64_t T, const int32_t *__restrict__ te,
+                             int64_t E, const int64_t *__restrict__ et_off, const int32_t *__restrict__ et,
+                             int32_t *__restrict__ ffirst, uint8_t *__restrict__ flags,
+                             int32_t *__restrict__ err) {
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < 4 * T;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        int32_t nm;
+        const int32_t f = face_first(tv4, T, te, E, et_off, et, (int32_t)(p >> 2), (int)(p & 3),
+                                     nm, nullptr);
+        // the cell itself always matches; no match means ET and TE disagree
+        const bool bad = f > (int32_t)p;
+        if (bad) *err = 1;
+        ffirst[p] = bad ? (int32_t)p : f;
+        flags[p] = bad || f == (int32_t)p;
+    }
+}
+
+// TF; FV as (mid, hi, lo) -- the fields (middleVert, highVert, low vertex);
+// FT (<= 2 tets, ascending, -1 pad)
+__global__ void k_face_ids(const int4 *__restrict__ tv4, int64_t T, const int32_t *__restrict__ te,
+                           int64_t E, const int64_t *__restrict__ et_off, const int32_t *__restrict__ et,
+                           const int32_t *__restrict__ ffirst, const uint32_t *__restrict__ bits,
+                           const int32_t *__restrict__ wpre, int32_t *__restrict__ tf,
+                           int32_t *__restrict__ fv, int32_t *__restrict__ ft,
+                           int32_t *__restrict__ err) {
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < 4 * T;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t f = ffirst[p];
+        const int32_t id = rank_of(bits, wpre, f);
+        tf[p] = id;
+        if (f != (int32_t)p) continue;
+        const int32_t t = (int32_t)(p >> 2);
+        const int s = (int)(p & 3);
+        const int4 q = tv4[t];
+        int lo, mid, hi;
+        face_pos(s, lo, mid, hi);
+        fv[3 * (int64_t)id] = corner(q, mid);
+        fv[3 * (int64_t)id + 1] = corner(q, hi);
+        fv[3 * (int64_t)id + 2] = corner(q, lo);
+        int32_t nm, two[2] = {-1, -1};
+        face_first(tv4, T, te, E, et_off, et, t, s, nm, two);
+        if (nm > 2) *err = 1;  // every writer stores the same value
+        ft[2 * (int64_t)id] = two[0];
+        ft[2 * (int64_t)id + 1] = nm >= 2 ? two[1] : -1;
+    }
+}
+
+// VF rows {mid, hi, id}: the row of a holds the faces whose positional low
+// vertex is a.  Lane per vertex walks its tets ascending and takes the slots
+// where a sits at the low position and which are their face's first
+// occurrence; positions ascend, so ids ascend.
+template <bool FILL>
+__global__ void k_vf(const int4 *__restrict__ tv4, const int32_t *__restrict__ vt_off,
+                     const int32_t *__restrict__ vtc, int64_t V, const int32_t *__restrict__ ffirst,
+                     const int32_t *__restrict__ tf, int32_t *__restrict__ cnt_or_off,
+                     int32_t *__restrict__ out) {
+    for (int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; a < V;
+         a += (int64_t)gridDim.x * blockDim.x) {
+        int32_t *row = FILL ? out + 3 * (int64_t)cnt_or_off[a] : nullptr;
+        int32_t n = 0;
+        for (int32_t k = vt_off[a]; k < vt_off[a + 1]; k++) {
+            const int32_t p = vtc[k];
+            const int32_t t = p >> 2, j = p & 3;
+            if (j > 1) continue;  // position 0 is low in slots 0-2, position 1 in slot 3
+            const int4 q = tv4[t];
+            for (int s = (j == 0 ? 0 : 3); s <= (j == 0 ? 2 : 3); s++) {
+                const int64_t pos = 4 * (int64_t)t + s;
+                if (ffirst[pos] != (int32_t)pos) continue;
+                if (FILL) {
+                    int lo, mid, hi;
+                    face_pos(s, lo, mid, hi);
+                    row[3 * n] = corner(q, mid);
+                    row[3 * n + 1] = corner(q, hi);
+                    row[3 * n + 2] = tf[pos];
+                }
+                n++;
+            }
+        }
+        if (!FILL) cnt_or_off[a] = n;
+    }
+}
+
+// FE[f] = {max vertex, edge id} for the edges (lo,mid), (lo,hi), (mid,hi)
+__global__ void k_fe(const int32_t *__restrict__ fv, int64_t F, const int32_t *__restrict__ vv_off,
+                     const int32_t *__restrict__ vv, const int32_t *__restrict__ vv_eid,
+                     int32_t *__restrict__ fe) {
+    for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < F;
+         f += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t mid = fv[3 * f], hi = fv[3 * f + 1], lo = fv[3 * f + 2];
+        const int32_t pa[3] = {lo, lo, mid}, pb[3] = {mid, hi, hi};
+        for (int e = 0; e < 3; e++) {
+            fe[6 * f + 2 * e] = max(pa[e], pb[e]);
+            fe[6 * f + 2 * e + 1] = edge_id(vv_off, vv, vv_eid, pa[e], pb[e]);
+        }
+    }
+}
+
+// insert x into the ascending row[0..m) unless present
+__device__ __forceinline__ void insert_sorted(int32_t *row, int64_t &m, int32_t x) {
+    int64_t i = m;
+    while (i > 0 && row[i - 1] > x) i--;
+    if (i > 0 && row[i - 1] == x) return;
+    for (int64_t k = m; k > i; k--) row[k] = row[k - 1];
+    row[i] = x;
+    m++;
+}
+
+// EF(e): faces holding e, from the two faces of e in each cell of ET(e)
+template <bool FILL>
+__global__ void k_ef(const int32_t *__restrict__ ev, int64_t E, const int64_t *__restrict__ et_off,
+                     const int32_t *__restrict__ et, const int32_t *__restrict__ te,
+                     const int32_t *__restrict__ tf, int64_t *__restrict__ cnt_or_off,
+                     int32_t *__restrict__ out, int32_t *__restrict__ tmp) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        int32_t *row = FILL ? out + cnt_or_off[e] : tmp + 2 * et_off[e];
+        int64_t m = 0;
+        for (int64_t i = et_off[e]; i < et_off[e + 1]; i++) {
+            const int32_t t = et[i];
+            int k = 0;
+            for (; k < 6; k++)
+                if (te[6 * (int64_t)t + k] == (int32_t)e) break;
+            // corner pair of edge slot k; faces holding both corners
+            const int pi[6] = {0, 0, 0, 1, 1, 2}, pj[6] = {1, 2, 3, 2, 3, 3};
+            for (int s = 0; s < 4; s++) {
+                int lo, mid, hi;
+                face_pos(s, lo, mid, hi);
+                const bool has_i = lo == pi[k] || mid == pi[k] || hi == pi[k];
+                const bool has_j = lo == pj[k] || mid == pj[k] || hi == pj[k];
+                if (has_i && has_j) insert_sorted(row, m, tf[4 * (int64_t)t + s]);
+            }
+        }
+        if (!FILL) cnt_or_off[e] = m;
+    }
+}
+
+// TT(t): union of VT rows of t's corners, minus t
+template <bool FILL>
+__global__ void k_tt(const int4 *__restrict__ tv4, int64_t T, const int32_t *__restrict__ vt_off,
+                     const int32_t *__restrict__ vtc, int64_t *__restrict__ cnt_or_off,
+                     int32_t *__restrict__ out) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < T;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int4 q = tv4[t];
+        int32_t pos[4], end[4];
+        for (int j = 0; j < 4; j++) {
+            const int32_t v = corner(q, j);
+            pos[j] = vt_off[v];
+            end[j] = vt_off[v + 1];
+        }
+        int64_t n = 0, o = FILL ? cnt_or_off[t] : 0;
+        // 4-way merge with duplicate removal
+        while (true) {
+            int32_t mn = INF;
+            for (int j = 0; j < 4; j++)
+                if (pos[j] < end[j]) mn = min(mn, vtc[pos[j]] >> 2);
+            if (mn == INF) break;
+            for (int j = 0; j < 4; j++)
+                if (pos[j] < end[j] && (vtc[pos[j]] >> 2) == mn) pos[j]++;
+            if (mn == (int32_t)t) continue;
+            if (FILL) out[o + n] = mn;
+            n++;
+        }
+        if (!FILL) cnt_or_off[t] = n;
+    }
+}
+
+// FF(f): faces of the cells holding f, minus f
+template <bool FILL>
+__global__ void k_ff(const int32_t *__restrict__ ft, int64_t F, const int32_t *__restrict__ tf,
+                     int64_t *__restrict__ cnt_or_off, int32_t *__restrict__ out) {
+    for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < F;
+         f += (int64_t)gridDim.x * blockDim.x) {
+        int32_t c[8];
+        int n = 0;
+        for (int k = 0; k < 2; k++) {
+            const int32_t t = ft[2 * f + k];
+            if (t < 0) continue;
+            for (int s = 0; s < 4; s++) {
+                const int32_t g = tf[4 * (int64_t)t + s];
+                if (g == (int32_t)f) continue;
+                int i = n - 1;
+                bool dup = false;
+                for (int z = 0; z < n; z++) dup |= c[z] == g;
+                if (dup) continue;
+                while (i >= 0 && c[i] > g) {
+                    c[i + 1] = c[i];
+                    i--;
+                }
+                c[i + 1] = g;
+                n++;
+            }
+        }
+        if (FILL) {
+            const int64_t o = cnt_or_off[f];
+            for (int i = 0; i < n; i++) out[o + i] = c[i];
+        } else {
+            cnt_or_off[f] = n;
+        }
+    }
+}
+
+// EE(e): edges at lo or hi, minus e, ascending (rows of vv_eid, sorted here)
+template <bool FILL>
+__global__ void k_ee(const int32_t *__restrict__ ev, int64_t E, const int32_t *__restrict__ vv_off,
+                     const int32_t *__restrict__ vv_eid, int64_t *__restrict__ cnt_or_off,
+                     int32_t *__restrict__ out) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t a = ev[2 * e], b = ev[2 * e + 1];
+        const int64_t n = (vv_off[a + 1] - vv_off[a]) + (vv_off[b + 1] - vv_off[b]) - 2;
+        if (!FILL) {
+            cnt_or_off[e] = n;
+            continue;
+        }
+        int32_t *row = out + cnt_or_off[e];
+        int64_t m = 0;
+        for (int side = 0; side < 2; side++) {
+            const int32_t v = side ? b : a;
+            for (int32_t i = vv_off[v]; i < vv_off[v + 1]; i++) {
+                const int32_t x = vv_eid[i];
+                if (x == (int32_t)e) continue;
+                int64_t k = m - 1;
+                while (k >= 0 && row[k] > x) {
+                    row[k + 1] = row[k];
+                    k--;
+                }
+                row[k + 1] = x;
+                m++;
+            }
+        }
+    }
+}
+
+__global__ void k_fill_i32v(int32_t *p, int32_t v, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_widen(const int32_t *__restrict__ in, int64_t n, int64_t *__restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = in[i];
+}
+
+unsigned grid(int64_t n, int block = 256) {
+    return std::min<unsigned>(tg::grid_for(n, block), 65536u);
+}
+
+int fail(int code, const char *msg) {
+    tg::set_error(msg);
+    return code;
+}
+
+int read_i32(tg_mesh &m, const void *src, int64_t index, int32_t &out) {
+    int32_t *h = (int32_t *)m.h_pinned;
+    TG_HIP(hipMemcpyAsync(h, (const int32_t *)src + index, sizeof(int32_t),
+                          hipMemcpyDeviceToHost, m.stream));
+    TG_HIP(hipStreamSynchronize(m.stream));
+    out = *h;
+    return tg::OK;
+}
+
+int read_i64(tg_mesh &m, const void *src, int64_t index, int64_t &out) {
+    int64_t *h = (int64_t *)m.h_pinned;
+    TG_HIP(hipMemcpyAsync(h, (const int64_t *)src + index, sizeof(int64_t),
+                          hipMemcpyDeviceToHost, m.stream));
+    TG_HIP(hipStreamSynchronize(m.stream));
+    out = *h;
+    return tg::OK;
+}
+
+// rank structure over n flags: bits + exclusive prefix of word popcounts
+int build_ranks(tg_mesh &m, const uint8_t *flags, int64_t n, int32_t *&bits, int32_t *&wpre,
+                int64_t &total) {
+    const int64_t nw = (n + 31) / 32;
+    int rc;
+    if ((rc = m.bitmap.ensure(sizeof(uint32_t) * (size_t)std::max<int64_t>(nw, 1))) ||
+        (rc = m.rank_tmp.ensure(sizeof(int32_t) * (size_t)(nw + 1))) ||
+        (rc = m.scan_tmp.ensure(tg::scan_tmp_bytes(nw))))
+        return rc;
+    bits = m.bitmap.as<int32_t>();
+    wpre = m.rank_tmp.as<int32_t>();
+    if (nw > 0) {
+        hipLaunchKernelGGL(k_pack_flags, dim3(grid(nw)), dim3(256), 0, m.stream, flags, n,
+                           (uint32_t *)bits, wpre);
+        TG_LAUNCH_CHECK("k_pack_flags");
+    }
+    if ((rc = tg::scan_exclusive_i32(wpre, wpre, nw, m.scan_tmp.p, m.stream))) return rc;
+    int32_t t32 = 0;
+    if ((rc = read_i32(m, wpre, nw, t32))) return rc;
+    total = t32;
+    return tg::OK;
+}
+
+// count -> exclusive scan (int64) -> offsets; returns nnz
+int scan_counts64(tg_mesh &m, int64_t *cnt_off, int64_t n, int64_t &nnz) {
+    int rc = m.scan_tmp.ensure(tg::scan_tmp_bytes(n));
+    if (rc) return rc;
+    if ((rc = tg::scan_exclusive_i64(cnt_off, cnt_off, n, m.scan_tmp.p, m.stream))) return rc;
+    return read_i64(m, cnt_off, n, nnz);
+}
+
+}  // namespace
 
 namespace tg {
-static int todo() {
-    set_error("relation not implemented yet");
-    return ERR_STATE;
+
+int ensure_edges(tg_mesh &m) {
+    if (m.edges_built) return OK;
+    int rc;
+    if ((rc = ensure_vv(m)) || (rc = ensure_vt_sorted(m)) || (rc = sync_and_check(m))) return rc;
+    const int64_t V = m.n_points, T = m.n_cells;
+    int32_t nnz = 0;
+    if ((rc = read_i32(m, m.vv_off.p, V, nnz))) return rc;
+    const size_t I = sizeof(int32_t);
+    if ((rc = m.edge_first.ensure(I * (size_t)std::max(nnz, 1))) ||
+        (rc = m.vv_eid.ensure(I * (size_t)std::max(nnz, 1))) ||
+        (rc = m.tmp_a.ensure(std::max<size_t>(6 * (size_t)T, 1))) ||  // flags
+        (rc = m.ve_off.ensure(I * (size_t)(V + 1))))
+        return rc;
+    uint8_t *flags = m.tmp_a.as<uint8_t>();
+    if ((rc = fill_u8(flags, 0, 6 * T, m.stream))) return rc;
+    hipLaunchKernelGGL(k_fill_i32v, dim3(grid(nnz)), dim3(256), 0, m.stream,
+                       m.edge_first.as<int32_t>(), INF, (int64_t)nnz);
+    TG_LAUNCH_CHECK("k_fill_i32v");
+    if (V > 0) {
+        hipLaunchKernelGGL(k_edge_first, dim3(grid(V)), dim3(256), 0, m.stream,
+                           m.tv.as<const int4>(), m.vt_off.as<int32_t>(), m.vtc.as<int32_t>(),
+                           m.vv_off.as<int32_t>(), m.vv_cols.as<int32_t>(), V,
+                           m.edge_first.as<int32_t>(), flags);
+        TG_LAUNCH_CHECK("k_edge_first");
+    }
+    int32_t *bits, *wpre;
+    int64_t E = 0;
+    if ((rc = build_ranks(m, flags, 6 * T, bits, wpre, E))) return rc;
+    m.n_edges = E;
+    if ((rc = m.ev.ensure(I * 2 * (size_t)std::max<int64_t>(E, 1))) ||
+        (rc = m.ve_vals.ensure(I * 2 * (size_t)std::max<int64_t>(E, 1))) ||
+        (rc = m.te.ensure(I * 6 * (size_t)std::max<int64_t>(T, 1))))
+        return rc;
+    const int4 *tv4 = m.tv.as<const int4>();
+    int32_t *voff = m.ve_off.as<int32_t>();
+    if (V > 0) {
+        hipLaunchKernelGGL((k_ve<false>), dim3(grid(V)), dim3(256), 0, m.stream, tv4,
+                           m.vt_off.as<int32_t>(), m.vtc.as<int32_t>(), V, flags,
+                           (const uint32_t *)bits, wpre, voff, nullptr);
+        TG_LAUNCH_CHECK("k_ve count");
+    }
+    if ((rc = scan_exclusive_i32(voff, voff, V, m.scan_tmp.p, m.stream))) return rc;
+    if (V > 0) {
+        hipLaunchKernelGGL((k_ve<true>), dim3(grid(V)), dim3(256), 0, m.stream, tv4,
+                           m.vt_off.as<int32_t>(), m.vtc.as<int32_t>(), V, flags,
+                           (const uint32_t *)bits, wpre, voff, m.ve_vals.as<int32_t>());
+        TG_LAUNCH_CHECK("k_ve fill");
+        hipLaunchKernelGGL(k_edge_ids, dim3(grid(V)), dim3(256), 0, m.stream,
+                           m.vv_off.as<int32_t>(), m.vv_cols.as<int32_t>(), V,
+                           m.edge_first.as<int32_t>(), (const uint32_t *)bits, wpre,
+                           m.vv_eid.as<int32_t>(), m.ev.as<int32_t>());
+        TG_LAUNCH_CHECK("k_edge_ids");
+        hipLaunchKernelGGL(k_edge_lower, dim3(grid(V)), dim3(256), 0, m.stream,
+                           m.vv_off.as<int32_t>(), m.vv_cols.as<int32_t>(), V,
+                           m.vv_eid.as<int32_t>());
+        TG_LAUNCH_CHECK("k_edge_lower");
+    }
+    if (T > 0) {
+        hipLaunchKernelGGL(k_te, dim3(grid(T)), dim3(256), 0, m.stream, m.tv.as<const int4>(), T,
+                           m.vv_off.as<int32_t>(), m.vv_cols.as<int32_t>(),
+                           m.vv_eid.as<int32_t>(), m.te.as<int32_t>());
+        TG_LAUNCH_CHECK("k_te");
+    }
+    m.edges_built = true;
+    return OK;
 }
-int ensure_edges(tg_mesh &) { return todo(); }
-int ensure_faces(tg_mesh &) { return todo(); }
-int ensure_et(tg_mesh &) { return todo(); }
-int ensure_fe(tg_mesh &) { return todo(); }
-int ensure_ft(tg_mesh &) { return todo(); }
-int ensure_ef(tg_mesh &) { return todo(); }
-int ensure_tt(tg_mesh &) { return todo(); }
-int ensure_ff(tg_mesh &) { return todo(); }
-int ensure_ee(tg_mesh &) { return todo(); }
-int id_relation_view(tg_mesh &, int32_t, RelView &) { return todo(); }
+
+int ensure_et(tg_mesh &m) {
+    if (m.et_built) return OK;
+    int rc;
+    // the vertex-lane VV form rewrites the corner lists unsorted: sort again
+    if ((rc = ensure_edges(m)) || (rc = ensure_vt_sorted(m))) return rc;
+    const int64_t E = m.n_edges;
+    if ((rc = m.et_off.ensure(sizeof(int64_t) * (size_t)(E + 1)))) return rc;
+    int64_t *off = m.et_off.as<int64_t>();
+    if (E > 0) {
+        hipLaunchKernelGGL((k_et<false>), dim3(grid(E)), dim3(256), 0, m.stream,
+                           m.ev.as<int32_t>(), E, m.n_points, m.vt_off.as<int32_t>(),
+                           m.vtc.as<int32_t>(), off, nullptr);
+        TG_LAUNCH_CHECK("k_et count");
+    }
+    int64_t nnz = 0;
+    if ((rc = scan_counts64(m, off, E, nnz))) return rc;
+    if ((rc = m.et_vals.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)))) return rc;
+    if (E > 0) {
+        hipLaunchKernelGGL((k_et<true>), dim3(grid(E)), dim3(256), 0, m.stream,
+                           m.ev.as<int32_t>(), E, m.n_points, m.vt_off.as<int32_t>(),
+                           m.vtc.as<int32_t>(), off, m.et_vals.as<int32_t>());
+        TG_LAUNCH_CHECK("k_et fill");
+    }
+    m.et_built = true;
+    return OK;
+}
+
+int ensure_faces(tg_mesh &m) {
+    if (m.faces_built) return OK;
+    int rc;
+    if ((rc = ensure_et(m)) || (rc = ensure_vt_sorted(m))) return rc;
+    const int64_t V = m.n_points, T = m.n_cells;
+    const size_t I = sizeof(int32_t);
+    if ((rc = m.tmp_b.ensure(I * std::max<size_t>(4 * (size_t)T, 1))) ||  // ffirst
+        (rc = m.tmp_a.ensure(std::max<size_t>(4 * (size_t)T, 1))) ||      // flags
+        (rc = m.tf.ensure(I * std::max<size_t>(4 * (size_t)T, 1))))
+        return rc;
+    int32_t *ffirst = m.tmp_b.as<int32_t>();
+    uint8_t *flags = m.tmp_a.as<uint8_t>();
+    int32_t *err = m.counters.as<int32_t>() + C_FTERR;
+    int32_t *iderr = m.counters.as<int32_t>() + C_IDERR;
+    if ((rc = fill_i32(err, 0, 2, m.stream))) return rc;
+    if (T > 0) {
+        hipLaunchKernelGGL(k_face_first, dim3(grid(4 * T)), dim3(256), 0, m.stream,
+                           m.tv.as<const int4>(), T, m.te.as<int32_t>(), m.n_edges,
+                           m.et_off.as<int64_t>(), m.et_vals.as<int32_t>(), ffirst, flags, iderr);
+        TG_LAUNCH_CHECK("k_face_first");
+    }
+    int32_t *bits, *wpre;
+    int64_t F = 0;
+    if ((rc = build_ranks(m, flags, 4 * T, bits, wpre, F))) return rc;
+    m.n_faces = F;
+    if ((rc = m.fv.ensure(I * 3 * (size_t)std::max<int64_t>(F, 1))) ||
+        (rc = m.ft.ensure(I * 2 * (size_t)std::max<int64_t>(F, 1))) ||
+        (rc = m.face_off.ensure(I * (size_t)(V + 1))))
+        return rc;
+    if (T > 0) {
+        hipLaunchKernelGGL(k_face_ids, dim3(grid(4 * T)), dim3(256), 0, m.stream,
+                           m.tv.as<const int4>(), T, m.te.as<int32_t>(), m.n_edges,
+                           m.et_off.as<int64_t>(), m.et_vals.as<int32_t>(), ffirst,
+                           (const uint32_t *)bits, wpre,
+                           m.tf.as<int32_t>(), m.fv.as<int32_t>(), m.ft.as<int32_t>(), err);
+        TG_LAUNCH_CHECK("k_face_ids");
+    }
+    int32_t ferr = 0, ierr = 0;
+    if ((rc = read_i32(m, err, 0, ferr)) || (rc = read_i32(m, iderr, 0, ierr))) return rc;
+    if (ierr) return fail(ERR_STATE, "face ids: TE and ET disagree (a cell is missing from ET)");
+    m.ft_ok = ferr == 0;
+    int32_t *voff = m.face_off.as<int32_t>();
+    if (V > 0) {
+        hipLaunchKernelGGL((k_vf<false>), dim3(grid(V)), dim3(256), 0, m.stream,
+                           m.tv.as<const int4>(), m.vt_off.as<int32_t>(), m.vtc.as<int32_t>(), V,
+                           ffirst, m.tf.as<int32_t>(), voff, nullptr);
+        TG_LAUNCH_CHECK("k_vf count");
+    }
+    if ((rc = scan_exclusive_i32(voff, voff, V, m.scan_tmp.p, m.stream))) return rc;
+    if ((rc = m.vf_vals.ensure(I * 3 * (size_t)std::max<int64_t>(F, 1)))) return rc;
+    if (V > 0) {
+        hipLaunchKernelGGL((k_vf<true>), dim3(grid(V)), dim3(256), 0, m.stream,
+                           m.tv.as<const int4>(), m.vt_off.as<int32_t>(), m.vtc.as<int32_t>(), V,
+                           ffirst, m.tf.as<int32_t>(), voff, m.vf_vals.as<int32_t>());
+        TG_LAUNCH_CHECK("k_vf fill");
+    }
+    m.faces_built = true;
+    return OK;
+}
+
+int ensure_fe(tg_mesh &m) {
+    if (m.fe_built) return OK;
+    int rc;
+    if ((rc = ensure_edges(m)) || (rc = ensure_faces(m))) return rc;
+    const int64_t F = m.n_faces;
+    if ((rc = m.fe.ensure(sizeof(int32_t) * 6 * (size_t)std::max<int64_t>(F, 1)))) return rc;
+    if (F > 0) {
+        hipLaunchKernelGGL(k_fe, dim3(grid(F)), dim3(256), 0, m.stream, m.fv.as<int32_t>(), F,
+                           m.vv_off.as<int32_t>(), m.vv_cols.as<int32_t>(),
+                           m.vv_eid.as<int32_t>(), m.fe.as<int32_t>());
+        TG_LAUNCH_CHECK("k_fe");
+    }
+    m.fe_built = true;
+    return OK;
+}
+
+int ensure_ft(tg_mesh &m) {
+    int rc = ensure_faces(m);
+    if (rc) return rc;
+    if (!m.ft_ok) return fail(ERR_MESH, "a face is held by more than two cells; FT needs <= 2");
+    return OK;
+}
+
+int ensure_ef(tg_mesh &m) {
+    if (m.ef_built) return OK;
+    int rc;
+    if ((rc = ensure_et(m)) || (rc = ensure_faces(m))) return rc;
+    const int64_t E = m.n_edges;
+    int64_t et_nnz = 0;
+    if ((rc = read_i64(m, m.et_off.p, E, et_nnz))) return rc;
+    if ((rc = m.ef_off.ensure(sizeof(int64_t) * (size_t)(E + 1))) ||
+        (rc = m.tmp_c.ensure(sizeof(int32_t) * 2 * (size_t)std::max<int64_t>(et_nnz, 1))))
+        return rc;
+    int64_t *off = m.ef_off.as<int64_t>();
+    if (E > 0) {
+        hipLaunchKernelGGL((k_ef<false>), dim3(grid(E)), dim3(256), 0, m.stream,
+                           m.ev.as<int32_t>(), E, m.et_off.as<int64_t>(), m.et_vals.as<int32_t>(),
+                           m.te.as<int32_t>(), m.tf.as<int32_t>(), off, nullptr,
+                           m.tmp_c.as<int32_t>());
+        TG_LAUNCH_CHECK("k_ef count");
+    }
+    int64_t nnz = 0;
+    if ((rc = scan_counts64(m, off, E, nnz))) return rc;
+    if ((rc = m.ef_vals.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)))) return rc;
+    if (E > 0) {
+        hipLaunchKernelGGL((k_ef<true>), dim3(grid(E)), dim3(256), 0, m.stream,
+                           m.ev.as<int32_t>(), E, m.et_off.as<int64_t>(), m.et_vals.as<int32_t>(),
+                           m.te.as<int32_t>(), m.tf.as<int32_t>(), off, m.ef_vals.as<int32_t>(),
+                           nullptr);
+        TG_LAUNCH_CHECK("k_ef fill");
+    }
+    m.ef_nnz = nnz;
+    m.ef_built = true;
+    return OK;
+}
+
+int ensure_tt(tg_mesh &m) {
+    if (m.tt_built) return OK;
+    int rc;
+    if ((rc = ensure_vt_sorted(m))) return rc;
+    const int64_t T = m.n_cells;
+    if ((rc = m.tt_off.ensure(sizeof(int64_t) * (size_t)(T + 1)))) return rc;
+    int64_t *off = m.tt_off.as<int64_t>();
+    if (T > 0) {
+        hipLaunchKernelGGL((k_tt<false>), dim3(grid(T)), dim3(256), 0, m.stream,
+                           m.tv.as<const int4>(), T, m.vt_off.as<int32_t>(), m.vtc.as<int32_t>(),
+                           off, nullptr);
+        TG_LAUNCH_CHECK("k_tt count");
+    }
+    int64_t nnz = 0;
+    if ((rc = scan_counts64(m, off, T, nnz))) return rc;
+    if ((rc = m.tt_vals.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)))) return rc;
+    if (T > 0) {
+        hipLaunchKernelGGL((k_tt<true>), dim3(grid(T)), dim3(256), 0, m.stream,
+                           m.tv.as<const int4>(), T, m.vt_off.as<int32_t>(), m.vtc.as<int32_t>(),
+                           off, m.tt_vals.as<int32_t>());
+        TG_LAUNCH_CHECK("k_tt fill");
+    }
+    m.tt_nnz = nnz;
+    m.tt_built = true;
+    return OK;
+}
+
+int ensure_ff(tg_mesh &m) {
+    if (m.ff_built) return OK;
+    int rc;
+    if ((rc = ensure_ft(m))) return rc;
+    const int64_t F = m.n_faces;
+    if ((rc = m.ff_off.ensure(sizeof(int64_t) * (size_t)(F + 1)))) return rc;
+    int64_t *off = m.ff_off.as<int64_t>();
+    if (F > 0) {
+        hipLaunchKernelGGL((k_ff<false>), dim3(grid(F)), dim3(256), 0, m.stream,
+                           m.ft.as<int32_t>(), F, m.tf.as<int32_t>(), off, nullptr);
+        TG_LAUNCH_CHECK("k_ff count");
+    }
+    int64_t nnz = 0;
+    if ((rc = scan_counts64(m, off, F, nnz))) return rc;
+    if ((rc = m.ff_vals.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)))) return rc;
+    if (F > 0) {
+        hipLaunchKernelGGL((k_ff<true>), dim3(grid(F)), dim3(256), 0, m.stream,
+                           m.ft.as<int32_t>(), F, m.tf.as<int32_t>(), off, m.ff_vals.as<int32_t>());
+        TG_LAUNCH_CHECK("k_ff fill");
+    }
+    m.ff_nnz = nnz;
+    m.ff_built = true;
+    return OK;
+}
+
+int ensure_ee(tg_mesh &m) {
+    if (m.ee_built) return OK;
+    int rc;
+    if ((rc = ensure_edges(m))) return rc;
+    const int64_t E = m.n_edges;
+    if ((rc = m.ee_off.ensure(sizeof(int64_t) * (size_t)(E + 1)))) return rc;
+    int64_t *off = m.ee_off.as<int64_t>();
+    if (E > 0) {
+        hipLaunchKernelGGL((k_ee<false>), dim3(grid(E)), dim3(256), 0, m.stream,
+                           m.ev.as<int32_t>(), E, m.vv_off.as<int32_t>(), m.vv_eid.as<int32_t>(),
+                           off, nullptr);
+        TG_LAUNCH_CHECK("k_ee count");
+    }
+    int64_t nnz = 0;
+    if ((rc = scan_counts64(m, off, E, nnz))) return rc;
+    if ((rc = m.ee_vals.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)))) return rc;
+    if (E > 0) {
+        hipLaunchKernelGGL((k_ee<true>), dim3(grid(E)), dim3(256), 0, m.stream,
+                           m.ev.as<int32_t>(), E, m.vv_off.as<int32_t>(), m.vv_eid.as<int32_t>(),
+                           off, m.ee_vals.as<int32_t>());
+        TG_LAUNCH_CHECK("k_ee fill");
+    }
+    m.ee_nnz = nnz;
+    m.ee_built = true;
+    return OK;
+}
+
+int id_relation_view(tg_mesh &m, int32_t rel, RelView &r) {
+    const int64_t V = m.n_points, T = m.n_cells, E = m.n_edges, F = m.n_faces;
+    auto need = [&](bool ok, const char *what) { return ok ? OK : fail(ERR_STATE, what); };
+    int rc;
+    int64_t nnz = 0;
+    switch (rel) {
+    case TG_REL_TE:
+        if ((rc = need(m.edges_built, "TE not built"))) return rc;
+        r = {T, T, 6, nullptr, 0, m.te.as<int32_t>()};
+        return OK;
+    case TG_REL_EV:
+        if ((rc = need(m.edges_built, "EV not built"))) return rc;
+        r = {E, E, 2, nullptr, 0, m.ev.as<int32_t>()};
+        return OK;
+    case TG_REL_VE:
+        if ((rc = need(m.edges_built, "VE not built"))) return rc;
+        r = {V, E, 2, m.ve_off.p, 4, m.ve_vals.as<int32_t>()};
+        return OK;
+    case TG_REL_ET:
+        if ((rc = need(m.et_built, "ET not built"))) return rc;
+        if ((rc = read_i64(m, m.et_off.p, E, nnz))) return rc;
+        r = {E, nnz, 1, m.et_off.p, 8, m.et_vals.as<int32_t>()};
+        return OK;
+    case TG_REL_TF:
+        if ((rc = need(m.faces_built, "TF not built"))) return rc;
+        r = {T, T, 4, nullptr, 0, m.tf.as<int32_t>()};
+        return OK;
+    case TG_REL_FV:
+        if ((rc = need(m.faces_built, "FV not built"))) return rc;
+        r = {F, F, 3, nullptr, 0, m.fv.as<int32_t>()};
+        return OK;
+    case TG_REL_VF:
+        if ((rc = need(m.faces_built, "VF not built"))) return rc;
+        r = {V, F, 3, m.face_off.p, 4, m.vf_vals.as<int32_t>()};
+        return OK;
+    case TG_REL_FE:
+        if ((rc = need(m.fe_built, "FE not built"))) return rc;
+        r = {F, F, 6, nullptr, 0, m.fe.as<int32_t>()};
+        return OK;
+    case TG_REL_FT:
+        if ((rc = need(m.faces_built, "FT not built"))) return rc;
+        if (!m.ft_ok) return fail(ERR_MESH, "a face is held by more than two cells");
+        r = {F, F, 2, nullptr, 0, m.ft.as<int32_t>()};
+        return OK;
+    case TG_REL_EF:
+        if ((rc = need(m.ef_built, "EF not built"))) return rc;
+        r = {E, m.ef_nnz, 1, m.ef_off.p, 8, m.ef_vals.as<int32_t>()};
+        return OK;
+    case TG_REL_TT:
+        if ((rc = need(m.tt_built, "TT not built"))) return rc;
+        r = {T, m.tt_nnz, 1, m.tt_off.p, 8, m.tt_vals.as<int32_t>()};
+        return OK;
+    case TG_REL_FF:
+        if ((rc = need(m.ff_built, "FF not built"))) return rc;
+        r = {F, m.ff_nnz, 1, m.ff_off.p, 8, m.ff_vals.as<int32_t>()};
+        return OK;
+    case TG_REL_EE:
+        if ((rc = need(m.ee_built, "EE not built"))) return rc;
+        r = {E, m.ee_nnz, 1, m.ee_off.p, 8, m.ee_vals.as<int32_t>()};
+        return OK;
+    default:
+        return fail(ERR_ARG, "unknown relation id");
+    }
+}
+
 }  // namespace tg

@@ -15,6 +15,9 @@ enum Counter : int {
     C_FIXBIG = 2,  // same, deg > 64
     C_ERR = 3,     // error bits (ERRBIT_*)
     C_MAXDEG = 4,  // largest VV degree seen
+    C_FIXD = 5,    // bucketed vertices needing the link pass (row descriptors)
+    C_FTERR = 6,   // a face held by more than two cells
+    C_IDERR = 7,   // face ids: a relation read back inconsistent
     N_COUNTERS = 8
 };
 enum ErrBit : int { ERRBIT_CAPACITY = 1 };
@@ -59,6 +62,10 @@ struct VvCpBuffers {
     uint32_t *classes = nullptr;
     int32_t *heavy = nullptr, *fix = nullptr, *fixbig = nullptr,
             *counters = nullptr;
+    // bucketed form: per-entry (lower bit, shared-tet component) of each row,
+    // in the padded row layout, and the vertices that use it
+    int32_t *fixd = nullptr, *rowsrc = nullptr;
+    uint8_t *rowdesc = nullptr;
     void *scan_tmp = nullptr;
     StageMarks *marks = nullptr;
 };

@@ -40,7 +40,7 @@ struct tg_mesh {
     // VV + critical point pipeline
     tg::DevArr vt_cnt, vt_off, vt_cur, vtc, scratch, deg, vv_off, vv_cols, classes,
         heavy, fix, fixbig, counters, scan_tmp, labels;
-    tg::DevArr bk_h1, bk_h2, bk_rowsrc;
+    tg::DevArr bk_h1, bk_h2, bk_rowsrc, bk_fixd, bk_rowdesc;
     tg::BucketPlan bplan;
     bool vt_ready = false, bk_ready = false;
     bool hot_ready = false;         // buffers sized for the hot path
@@ -58,6 +58,7 @@ struct tg_mesh {
     bool edges_built = false, faces_built = false, vt_built = false, et_built = false,
          fe_built = false, ft_built = false, ef_built = false, tt_built = false,
          ff_built = false, ee_built = false;
+    bool ft_ok = true;  // every face is held by at most two cells
     int64_t tt_nnz = 0, ff_nnz = 0, ee_nnz = 0, ef_nnz = 0;
 
     void *h_pinned = nullptr;  // small pinned staging area (64 int64)

@@ -42,7 +42,8 @@ void tg_mesh::release_all() {
                          &et_off,   &et_vals,  &face_off, &vf_vals,  &fv,       &tf,
                          &fe,       &ft,       &ef_off,   &ef_vals,  &tt_off,   &tt_vals,
                          &ff_off,   &ff_vals,  &ee_off,   &ee_vals,  &bitmap,   &rank_tmp,
-                         &tmp_a,    &tmp_b,    &tmp_c,    &bk_h1,    &bk_h2,    &bk_rowsrc};
+                         &tmp_a,    &tmp_b,    &tmp_c,    &bk_h1,    &bk_h2,    &bk_rowsrc,
+                         &bk_fixd,  &bk_rowdesc};
     for (auto *a : all) a->release();
     for (auto &e : marks.ev)
         if (e) {
@@ -115,6 +116,8 @@ int prepare_buckets(tg_mesh &m) {
     const size_t n1 = (size_t)p.nc * (size_t)p.nchunks + 1, n2 = (size_t)p.nb * p.slices + 1;
     if ((rc = m.bk_h1.ensure(I * n1)) || (rc = m.bk_h2.ensure(I * n2)) ||
         (rc = m.bk_rowsrc.ensure(I * (size_t)m.n_points)) ||
+        (rc = m.bk_fixd.ensure(I * (size_t)m.n_points)) ||
+        (rc = m.bk_rowdesc.ensure(12 * (size_t)m.n_cells)) ||
         (rc = m.scan_tmp.ensure(scan_tmp_bytes((int64_t)std::max(
              {n1, n2, (size_t)m.n_points, (size_t)(12 * m.n_cells)})))))
         return rc;
@@ -137,6 +140,9 @@ static BucketBuffers bucket_buffers(tg_mesh &m) {
     k.rec1 = m.scratch.as<uint32_t>();
     k.rec2 = m.vv_cols.as<uint32_t>();
     k.rowsrc = m.bk_rowsrc.as<int32_t>();
+    k.base.rowsrc = k.rowsrc;
+    k.base.fixd = m.bk_fixd.as<int32_t>();
+    k.base.rowdesc = m.bk_rowdesc.as<uint8_t>();
     return k;
 }
 

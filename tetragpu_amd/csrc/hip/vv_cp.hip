@@ -454,6 +454,24 @@ __device__ void link_components(int32_t v, const int32_t *__restrict__ off,
         }
 }
 
+__device__ __forceinline__ uint32_t n_components32(const uint32_t *adj, uint32_t mask) {
+    uint32_t cnt = 0, rem = mask;
+    while (rem) {
+        uint32_t comp = rem & (0u - rem);
+        uint32_t front = comp;
+        while (front) {
+            const int f = __ffs(front) - 1;
+            front &= front - 1u;
+            const uint32_t nb = adj[f] & mask & ~comp;
+            comp |= nb;
+            front |= nb;
+        }
+        rem &= ~comp;
+        cnt++;
+    }
+    return cnt;
+}
+
 __device__ __forceinline__ uint32_t n_components64(const uint64_t *adj, uint64_t mask) {
     uint32_t cnt = 0;
     uint64_t rem = mask;
@@ -534,6 +552,100 @@ __global__ __launch_bounds__(64) void k_cp_fix_wave(
     }
 }
 
+// Link pass driven by the row descriptors of the bucketed form: for a queued
+// vertex v (degree <= 32) each neighbour carries its class and the number of
+// its shared-tet component.  Only neighbours outside the largest component of
+// a multi-component class walk their rows; every row-membership edge between
+// two components of one class joins them, and the classes' counts are the
+// components of that component graph.  Two vertices per wave, 32 lanes each.
+__global__ __launch_bounds__(64) void k_cp_fix_desc(
+    const int32_t *__restrict__ off, const int32_t *__restrict__ cols,
+    const int32_t *__restrict__ rowsrc, const uint8_t *__restrict__ rowdesc,
+    const int32_t *__restrict__ fixd, int32_t *__restrict__ fix,
+    int32_t *__restrict__ counters, uint32_t *__restrict__ classes) {
+    __shared__ int32_t nbs[2][32];
+    __shared__ uint8_t dsc[2][32];
+    __shared__ uint32_t cadj[2][2][32];
+    const int lane = threadIdx.x & 31, half = threadIdx.x >> 5;
+    const int32_t n = counters[tg::C_FIXD];
+    const uint64_t hmask = half ? 0xFFFFFFFF00000000ull : 0xFFFFFFFFull;
+    for (int32_t i0 = 2 * blockIdx.x; i0 < n; i0 += 2 * gridDim.x) {
+        const int32_t i = i0 + half;
+        const bool live = i < n;
+        const int32_t v = live ? fixd[i] : 0;
+        const int32_t b = live ? off[v] : 0;
+        int d = live ? off[v + 1] - b : 0;
+        if (d > 32) {  // not from the bucketed kernel: generic pass
+            if (lane == 0) fix[atomicAdd(counters + C_FIX, 1)] = v;
+            d = 0;
+        }
+        const bool on = lane < d;
+        const int32_t a = on ? cols[b + lane] : INT32_MAX;
+        const uint32_t ds = on ? rowdesc[rowsrc[v] + lane] : 0u;
+        const int cls = (int)(ds >> 5), comp = (int)(ds & 31u);
+        nbs[half][lane] = a;
+        dsc[half][lane] = (uint8_t)ds;
+        cadj[half][0][lane] = 0u;
+        cadj[half][1][lane] = 0u;
+        int nc1 = on && cls == 1 ? comp + 1 : 0, nc0 = on && cls == 0 ? comp + 1 : 0;
+#pragma unroll
+        for (int s = 16; s >= 1; s >>= 1) {
+            nc1 = max(nc1, __shfl_xor(nc1, s, 32));
+            nc0 = max(nc0, __shfl_xor(nc0, s, 32));
+        }
+        // largest component of each multi-component class (lowest number on
+        // ties); the ballots run over the wave, each half reads its 32 bits
+        int big0 = 0, big1 = 0;
+        const int ncmax = max(__shfl(max(nc0, nc1), 0, 64), __shfl(max(nc0, nc1), 32, 64));
+        int best0 = -1, best1 = -1;
+        for (int j = 0; j < ncmax; j++) {
+            const int s0 = __popcll(__ballot(on && cls == 0 && comp == j) & hmask);
+            const int s1 = __popcll(__ballot(on && cls == 1 && comp == j) & hmask);
+            if (j < nc0 && s0 > best0) { best0 = s0; big0 = j; }
+            if (j < nc1 && s1 > best1) { best1 = s1; big1 = j; }
+        }
+        __syncthreads();
+        const int nck = cls ? nc1 : nc0;
+        const int big = cls ? big1 : big0;
+        if (on && nck >= 2 && comp != big) {
+            const int32_t rb = off[a], re = off[a + 1];
+            for (int32_t j0 = rb; j0 < re; j0 += 16) {
+                int32_t c[16];
+#pragma unroll
+                for (int k = 0; k < 16; k++) c[k] = j0 + k < re ? cols[j0 + k] : INT32_MAX;
+#pragma unroll
+                for (int k = 0; k < 16; k++) {
+                    int lo = 0, hi = d;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (nbs[half][mid] < c[k]) lo = mid + 1;
+                        else hi = mid;
+                    }
+                    if (lo < d && nbs[half][lo] == c[k]) {
+                        const int dk = dsc[half][lo];
+                        const int ck = dk & 31;
+                        if ((dk >> 5) == cls && ck != comp) {
+                            atomicOr(&cadj[half][cls][comp], 1u << ck);
+                            atomicOr(&cadj[half][cls][ck], 1u << comp);
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (lane == 0 && live && d > 0) {
+            const uint32_t m1 = nc1 >= 32 ? ~0u : ((1u << nc1) - 1u);
+            const uint32_t m0 = nc0 >= 32 ? ~0u : ((1u << nc0) - 1u);
+            const uint32_t nl = nc1 >= 2 ? n_components32(cadj[half][1], m1) : (uint32_t)nc1;
+            const uint32_t nu = nc0 >= 2 ? n_components32(cadj[half][0], m0) : (uint32_t)nc0;
+            classes[3 * (int64_t)v] = nl;
+            classes[3 * (int64_t)v + 1] = nu;
+            classes[3 * (int64_t)v + 2] = verdict(nl, nu);
+        }
+        __syncthreads();
+    }
+}
+
 template <typename S>
 __global__ __launch_bounds__(64) void k_cp_fix_big(
     const int32_t *__restrict__ off, const int32_t *__restrict__ cols,
@@ -564,6 +676,12 @@ int launch_cp_fix_typed(tg::VvCpBuffers &b, hipStream_t s) {
     const int32_t V = (int32_t)b.n_points;
     const S *scal = (const S *)b.scalars;
     const unsigned g = std::min<unsigned>(tg::grid_for(V, 1), 16384u);
+    if (b.fixd) {
+        hipLaunchKernelGGL(k_cp_fix_desc, dim3(std::max(1u, g / 2)), dim3(64), 0, s, b.vv_off,
+                           b.vv_cols, b.rowsrc,
+                           b.rowdesc, b.fixd, b.fix, b.counters, b.classes);
+        TG_LAUNCH_CHECK("k_cp_fix_desc");
+    }
     hipLaunchKernelGGL((k_cp_fix_wave<S>), dim3(g), dim3(64), 0, s, b.vv_off, b.vv_cols, scal,
                        b.fix, b.fixbig, b.counters, b.classes);
     TG_LAUNCH_CHECK("k_cp_fix_wave");

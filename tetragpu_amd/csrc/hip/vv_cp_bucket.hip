@@ -92,19 +92,26 @@ constexpr int MAX_FINE = 512;
 __global__ __launch_bounds__(L1_THREADS) void k_l1_count(const int4 *__restrict__ tv4, int64_t T,
                                                           int64_t tpc, int cshift, int32_t nc,
                                                           int32_t nch, int32_t *__restrict__ h1) {
-    __shared__ int32_t h[MAX_COARSE];
-    for (int i = threadIdx.x; i < nc; i += L1_THREADS) h[i] = 0;
+    constexpr int NW = L1_THREADS / 64;
+    __shared__ int32_t h[NW][MAX_COARSE];  // one sub-histogram per wave
+    int32_t *hw = h[threadIdx.x / 64];
+    for (int i = threadIdx.x; i < NW * MAX_COARSE; i += L1_THREADS) (&h[0][0])[i] = 0;
     __syncthreads();
     const int64_t t0 = (int64_t)blockIdx.x * tpc, t1 = min(T, t0 + tpc);
     for (int64_t t = t0 + threadIdx.x; t < t1; t += L1_THREADS) {
         const int4 q = tv4[t];
-        atomicAdd(&h[q.x >> cshift], 1);
-        atomicAdd(&h[q.y >> cshift], 1);
-        atomicAdd(&h[q.z >> cshift], 1);
-        atomicAdd(&h[q.w >> cshift], 1);
+        atomicAdd(&hw[q.x >> cshift], 1);
+        atomicAdd(&hw[q.y >> cshift], 1);
+        atomicAdd(&hw[q.z >> cshift], 1);
+        atomicAdd(&hw[q.w >> cshift], 1);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < nc; i += L1_THREADS) h1[(int64_t)i * nch + blockIdx.x] = h[i];
+    for (int i = threadIdx.x; i < nc; i += L1_THREADS) {
+        int32_t sum = 0;
+#pragma unroll
+        for (int w = 0; w < NW; w++) sum += h[w][i];
+        h1[(int64_t)i * nch + blockIdx.x] = sum;
+    }
 }
 
 // Staged scatter shared by both routing levels: a tile of records is
@@ -114,6 +121,7 @@ constexpr int STAGE = 4096;  // records per tile
 
 struct Stage {
     uint32_t w0[STAGE], w1[STAGE], w2[STAGE];
+    uint16_t bin[STAGE];
     int32_t cnt[MAX_FINE], toff[MAX_FINE], gbase[MAX_FINE];
     int32_t total;
 };
@@ -134,18 +142,13 @@ __device__ __forceinline__ void stage_scan(Stage &st, int nbin) {
     }
 }
 
-// Write the staged tile: slot -> bin via binary search over toff.
-__device__ __forceinline__ void stage_flush(Stage &st, int nbin, Rec out, int nthreads) {
+// Write the staged tile: consecutive slots of one bin go to consecutive
+// addresses of that bin's range.
+__device__ __forceinline__ void stage_flush(Stage &st, Rec out, int nthreads) {
     const int32_t total = st.total;
     for (int slot = threadIdx.x; slot < total; slot += nthreads) {
-        int lo = 0, hi = nbin - 1;  // last bin with toff <= slot and cnt > 0
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (st.toff[mid] <= slot) lo = mid;
-            else hi = mid - 1;
-        }
-        while (lo > 0 && st.cnt[lo] == 0) lo--;  // never taken: the last bin with toff <= slot is non-empty
-        const int64_t pos = (int64_t)st.gbase[lo] + (slot - st.toff[lo]);
+        const int b = st.bin[slot];
+        const int64_t pos = (int64_t)st.gbase[b] + (slot - st.toff[b]);
         out.w0[pos] = st.w0[slot];
         out.w1[pos] = st.w1[slot];
         out.w2[pos] = st.w2[slot];
@@ -161,15 +164,16 @@ __global__ __launch_bounds__(L1_THREADS) void k_l1_scatter(const int4 *__restric
     const int tid = threadIdx.x;
     for (int i = tid; i < nc; i += L1_THREADS) st.gbase[i] = h1s[(int64_t)i * nch + blockIdx.x];
     const int64_t t0 = (int64_t)blockIdx.x * tpc, t1 = min(T, t0 + tpc);
+    int4 qn = make_int4(0, 0, 0, 0);
+    if (tid < STAGE / 4 && t0 + tid < t1) qn = tv4[t0 + tid];
     for (int64_t tb = t0; tb < t1; tb += STAGE / 4) {
         for (int i = tid; i < nc; i += L1_THREADS) st.cnt[i] = 0;
         __syncthreads();
         const int64_t t = tb + tid;
         const bool on = tid < STAGE / 4 && t < t1;
-        int4 q = make_int4(0, 0, 0, 0);
+        const int4 q = qn;
         int r0 = 0, r1 = 0, r2 = 0, r3 = 0;
         if (on) {
-            q = tv4[t];
             r0 = atomicAdd(&st.cnt[q.x >> cshift], 1);
             r1 = atomicAdd(&st.cnt[q.y >> cshift], 1);
             r2 = atomicAdd(&st.cnt[q.z >> cshift], 1);
@@ -180,18 +184,22 @@ __global__ __launch_bounds__(L1_THREADS) void k_l1_scatter(const int4 *__restric
         __syncthreads();
         if (on) {
             const uint32_t x = q.x, y = q.y, z = q.z, w = q.w;
-            auto put = [&](int slot, uint32_t v, uint32_t a, uint32_t b, uint32_t c) {
+            auto put = [&](int bin, int rank, uint32_t v, uint32_t a, uint32_t b, uint32_t c) {
+                const int slot = st.toff[bin] + rank;
                 st.w0[slot] = v | (a << 24);
                 st.w1[slot] = (a >> 8) | (b << 16);
                 st.w2[slot] = (b >> 16) | (c << 8);
+                st.bin[slot] = (uint16_t)bin;
             };
-            put(st.toff[q.x >> cshift] + r0, x, y, z, w);
-            put(st.toff[q.y >> cshift] + r1, y, x, z, w);
-            put(st.toff[q.z >> cshift] + r2, z, x, y, w);
-            put(st.toff[q.w >> cshift] + r3, w, x, y, z);
+            put(q.x >> cshift, r0, x, y, z, w);
+            put(q.y >> cshift, r1, y, x, z, w);
+            put(q.z >> cshift, r2, z, x, y, w);
+            put(q.w >> cshift, r3, w, x, y, z);
         }
         __syncthreads();
-        stage_flush(st, nc, out, L1_THREADS);
+        // next tile's tets are in flight during the flush
+        if (tid < STAGE / 4 && tb + STAGE / 4 + tid < t1) qn = tv4[tb + STAGE / 4 + tid];
+        stage_flush(st, out, L1_THREADS);
         __syncthreads();
         for (int i = tid; i < nc; i += L1_THREADS) st.gbase[i] += st.cnt[i];
     }
@@ -211,19 +219,35 @@ __global__ __launch_bounds__(L2_THREADS) void k_l2_count(const uint32_t *__restr
                                                           const int32_t *__restrict__ h1s,
                                                           int32_t nch, int bv_log, int fs,
                                                           int32_t nb, int32_t *__restrict__ h2) {
-    __shared__ int32_t h[MAX_FINE];
+    constexpr int NSUB = 8;  // sub-histograms, two waves each
+    __shared__ int32_t h[NSUB][MAX_FINE];
     const int32_t c = blockIdx.x, s = blockIdx.y, S = gridDim.y;
     const int F = 1 << fs;
-    for (int i = threadIdx.x; i < F; i += L2_THREADS) h[i] = 0;
+    int32_t *hw = h[(threadIdx.x / 64) % NSUB];
+    for (int i = threadIdx.x; i < NSUB * MAX_FINE; i += L2_THREADS) (&h[0][0])[i] = 0;
     __syncthreads();
     int32_t lo, hi;
     slice_range(h1s, nch, c, s, S, lo, hi);
     const int32_t fb0 = c << fs;
-    for (int32_t i = lo + threadIdx.x; i < hi; i += L2_THREADS)
-        atomicAdd(&h[(int32_t)((w0[i] & M24) >> bv_log) - fb0], 1);
+    constexpr int U = 8;
+    for (int32_t i0 = lo + threadIdx.x; i0 < hi; i0 += U * L2_THREADS) {
+        uint32_t w[U];
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            const int32_t i = i0 + k * L2_THREADS;
+            w[k] = i < hi ? w0[i] : 0xFFFFFFFFu;
+        }
+#pragma unroll
+        for (int k = 0; k < U; k++)
+            if (w[k] != 0xFFFFFFFFu) atomicAdd(&hw[(int32_t)((w[k] & M24) >> bv_log) - fb0], 1);
+    }
     __syncthreads();
-    for (int i = threadIdx.x; i < F && fb0 + i < nb; i += L2_THREADS)
-        h2[(int64_t)(fb0 + i) * S + s] = h[i];
+    for (int i = threadIdx.x; i < F && fb0 + i < nb; i += L2_THREADS) {
+        int32_t sum = 0;
+#pragma unroll
+        for (int w = 0; w < NSUB; w++) sum += h[w][i];
+        h2[(int64_t)(fb0 + i) * S + s] = sum;
+    }
 }
 
 __global__ __launch_bounds__(L2_THREADS) void k_l2_scatter(const Rec in,
@@ -241,25 +265,29 @@ __global__ __launch_bounds__(L2_THREADS) void k_l2_scatter(const Rec in,
     int32_t lo, hi;
     slice_range(h1s, nch, c, s, S, lo, hi);
     constexpr int PER = STAGE / L2_THREADS;
-    for (int32_t ib = lo; ib < hi; ib += STAGE) {
-        for (int i = tid; i < F; i += L2_THREADS) st.cnt[i] = 0;
-        __syncthreads();
-        uint32_t a[PER], b[PER], d[PER];
-        int f[PER], r[PER];
+    uint32_t a[PER], b[PER], d[PER];
+    auto load = [&](int32_t ib) {
 #pragma unroll
         for (int k = 0; k < PER; k++) {
             const int32_t i = ib + k * L2_THREADS + tid;
-            f[k] = -1;
+            a[k] = 0xFFFFFFFFu;
             if (i < hi) {
                 a[k] = in.w0[i];
                 b[k] = in.w1[i];
                 d[k] = in.w2[i];
-                f[k] = (int)((a[k] & M24) >> bv_log) - fb0;
             }
         }
+    };
+    load(lo);
+    for (int32_t ib = lo; ib < hi; ib += STAGE) {
+        for (int i = tid; i < F; i += L2_THREADS) st.cnt[i] = 0;
+        __syncthreads();
+        int f[PER], r[PER];
 #pragma unroll
-        for (int k = 0; k < PER; k++)
+        for (int k = 0; k < PER; k++) {
+            f[k] = a[k] != 0xFFFFFFFFu ? (int)((a[k] & M24) >> bv_log) - fb0 : -1;
             if (f[k] >= 0) r[k] = atomicAdd(&st.cnt[f[k]], 1);
+        }
         __syncthreads();
         stage_scan(st, F);
         __syncthreads();
@@ -270,35 +298,63 @@ __global__ __launch_bounds__(L2_THREADS) void k_l2_scatter(const Rec in,
                 st.w0[slot] = a[k];
                 st.w1[slot] = b[k];
                 st.w2[slot] = d[k];
+                st.bin[slot] = (uint16_t)f[k];
             }
         __syncthreads();
-        stage_flush(st, F, out, L2_THREADS);
+        load(ib + STAGE);  // next tile's records are in flight during the flush
+        stage_flush(st, out, L2_THREADS);
         __syncthreads();
         for (int i = tid; i < F; i += L2_THREADS) st.gbase[i] += st.cnt[i];
     }
 }
 
 // ---- B: per-bucket neighbour sets, link components, sorted rows --------------
-__device__ __forceinline__ int hash_insert(int32_t *tab, uint32_t *occ, int32_t u) {
-    const uint32_t h = ((uint32_t)u * 2654435769u) >> 27;
+// Probe order of key u in a 32-slot table: 8 groups of 4 slots (one 16-byte
+// LDS read per group), starting at the home group from a multiplicative
+// hash; inside a group a key claims the first empty slot.
+__device__ __forceinline__ uint32_t key_hash(int32_t u) {
+    return ((uint32_t)u * 2654435769u) >> 27;  // 5 bits: group (3) | slot hint (2)
+}
+
+// First empty slot of a group snapshot at or after slot p (cyclic), or -1.
+__device__ __forceinline__ int first_empty_from(const int4 q, int p) {
+    const uint32_t m = (uint32_t)(q.x == -1) | ((uint32_t)(q.y == -1) << 1) |
+                       ((uint32_t)(q.z == -1) << 2) | ((uint32_t)(q.w == -1) << 3);
+    const uint32_t r = ((m | (m << 4)) >> p) & 15u;  // rotate so slot p is bit 0
+    return r ? (int)((__ffs(r) - 1 + p) & 3) : -1;
+}
+
+// Insert u into a vertex's table (lock-free: slots only go from empty (-1)
+// to a key by CAS, and every thread inserting u follows the same probe
+// order, so u lands in exactly one slot).  Returns the slot or -1 if full.
+__device__ __forceinline__ int hash_insert(int32_t *tab, int32_t u) {
+    const uint32_t h = key_hash(u);
 #pragma unroll 1
-    for (int p = 0; p < SLOTS; p++) {
-        const int s = (int)((h + (uint32_t)p) & (SLOTS - 1));
-        const int32_t cur = tab[s];
-        if (cur == u) return s;
-        if (cur == -1) {
-            const int32_t old = atomicCAS(&tab[s], -1, u);
-            if (old == -1) {
-                atomicOr(occ, 1u << s);
-                return s;
-            }
-            if (old == u) return s;
+    for (int step = 0; step < SLOTS / 4; step++) {
+        const int g = (int)(((h >> 2) + (uint32_t)step) & (SLOTS / 4 - 1));
+        int32_t *grp = tab + 4 * g;
+        int4 q = *reinterpret_cast<const int4 *>(grp);
+#pragma unroll 1
+        for (int tries = 0; tries < 4; tries++) {
+            if (q.x == u) return 4 * g;
+            if (q.y == u) return 4 * g + 1;
+            if (q.z == u) return 4 * g + 2;
+            if (q.w == u) return 4 * g + 3;
+            const int e = first_empty_from(q, 0);
+            if (e < 0) break;
+            const int32_t old = atomicCAS(&grp[e], -1, u);
+            if (old == -1 || old == u) return 4 * g + e;
+            q = *reinterpret_cast<const int4 *>(grp);  // lost to another key
         }
     }
     return -1;
 }
 
-__device__ __forceinline__ uint32_t n_components(const uint32_t *adj, uint32_t mask) {
+// Components of `mask` under the slot adjacency, numbered 0.. in order of
+// their lowest slot; the number of every slot's component is recorded in
+// five bit-planes (bit k of slot i's number is bit i of pl[k]).
+__device__ __forceinline__ uint32_t label_components(const uint32_t *adj, uint32_t mask,
+                                                     uint32_t (&pl)[5]) {
     uint32_t cnt = 0, rem = mask;
     while (rem) {
         uint32_t comp = rem & (0u - rem);
@@ -310,6 +366,9 @@ __device__ __forceinline__ uint32_t n_components(const uint32_t *adj, uint32_t m
             comp |= nb;
             front |= nb;
         }
+#pragma unroll
+        for (int k = 0; k < 5; k++)
+            if ((cnt >> k) & 1u) pl[k] |= comp;
         rem &= ~comp;
         cnt++;
     }
@@ -339,12 +398,13 @@ __global__ __launch_bounds__(2 << BV_LOG) void k_bkt_vv_cp(
     const Rec rec, const int32_t *__restrict__ h2s, int32_t nslice,
     const S *__restrict__ scal, int32_t V, int classify, int32_t *__restrict__ scratch,
     int32_t *__restrict__ deg, int32_t *__restrict__ rowsrc, uint32_t *__restrict__ classes,
-    int32_t *__restrict__ heavy, int32_t *__restrict__ fix, int32_t *__restrict__ counters) {
+    int32_t *__restrict__ heavy, int32_t *__restrict__ fixd, uint8_t *__restrict__ rowdesc,
+    int32_t *__restrict__ counters) {
     constexpr int BV = 1 << BV_LOG;
     constexpr int NT = 2 * BV;
     constexpr int BATCH = 4;
     constexpr int PER_T = BV * SLOTS / NT;  // slots per thread in phase 2 (16)
-    __shared__ int32_t key[BV * SLOTS];
+    __shared__ __align__(16) int32_t key[BV * SLOTS];
     __shared__ uint32_t adj[BV * SLOTS];
     __shared__ uint32_t occ[BV], lowm[BV];
     __shared__ int32_t ccount[BV];
@@ -371,38 +431,106 @@ __global__ __launch_bounds__(2 << BV_LOG) void k_bkt_vv_cp(
     }
     __syncthreads();
 
-    // phase 1: neighbour sets and shared-tet link adjacency (loads batched)
+    // phase 1: neighbour sets and shared-tet link adjacency.  For a batch of
+    // keys: all group reads are issued together; a key found in its group
+    // snapshot is placed, otherwise it claims the snapshot's first empty
+    // slot by CAS, and all those CAS are issued before any result is used,
+    // so a batch costs about two LDS round trips.  Keys whose CAS lost to a
+    // different key (or whose group was full) are queued in a per-lane mask
+    // and resolved by one shared probing loop.
     for (int32_t i0 = cbeg + tid; i0 < cend; i0 += BATCH * NT) {
+        constexpr int NK = 3 * BATCH;
         uint32_t w[BATCH][3];
 #pragma unroll
         for (int k = 0; k < BATCH; k++) {
             const int32_t i = i0 + k * NT;
+            w[k][0] = 0xFFFFFFFFu;
             if (i < cend) {
                 w[k][0] = rec.w0[i];
                 w[k][1] = rec.w1[i];
                 w[k][2] = rec.w2[i];
             }
         }
+        int32_t u[NK];
+        int vls[BATCH];
 #pragma unroll
         for (int k = 0; k < BATCH; k++) {
-            if (i0 + k * NT >= cend) break;
-            uint32_t v, x, y, z;
-            rec_unpack(w[k][0], w[k][1], w[k][2], v, x, y, z);
-            const int vl = (int)(v & (BV - 1));
-            int32_t *tab = key + vl * SLOTS;
+            uint32_t v = 0, x = 0, y = 0, z = 0;
+            if (w[k][0] != 0xFFFFFFFFu) rec_unpack(w[k][0], w[k][1], w[k][2], v, x, y, z);
+            vls[k] = (int)(v & (BV - 1));
+            u[3 * k] = (int32_t)x;
+            u[3 * k + 1] = (int32_t)y;
+            u[3 * k + 2] = (int32_t)z;
+        }
+        int4 grp[NK];
+#pragma unroll
+        for (int j = 0; j < NK; j++) {
+            const uint32_t g = key_hash(u[j]) >> 2;
+            grp[j] = *reinterpret_cast<const int4 *>(key + vls[j / 3] * SLOTS + 4 * g);
+        }
+        int sl[NK], cs[NK];
+#pragma unroll
+        for (int j = 0; j < NK; j++) {
+            const int32_t x = u[j];
+            const uint32_t h = key_hash(x);
+            const int g = (int)(h >> 2);
+            const int4 q = grp[j];
+            const int f = q.x == x ? 0 : q.y == x ? 1 : q.z == x ? 2 : q.w == x ? 3 : -1;
+            const int e = first_empty_from(q, 0);
+            const bool valid = w[j / 3][0] != 0xFFFFFFFFu;
+            sl[j] = f >= 0 ? 4 * g + f : -1;
+            cs[j] = (valid && f < 0 && e >= 0) ? 4 * g + e : -1;  // slot to claim
+            if (valid && f < 0 && e < 0) sl[j] = -2;            // group full
+        }
+        int32_t old[NK];
+#pragma unroll
+        for (int j = 0; j < NK; j++)
+            old[j] = cs[j] >= 0 ? atomicCAS(key + vls[j / 3] * SLOTS + cs[j], -1, u[j]) : 0;
+        uint32_t pending = 0;
+#pragma unroll
+        for (int j = 0; j < NK; j++) {
+            if (cs[j] >= 0) sl[j] = (old[j] == -1 || old[j] == u[j]) ? cs[j] : -2;
+            if (sl[j] == -2) pending |= 1u << j;
+        }
+        while (__any(pending != 0)) {
+            if (pending) {
+                const int j = __ffs(pending) - 1;
+                pending &= pending - 1u;
+                int32_t x = 0;
+                int vl = 0;
+#pragma unroll
+                for (int t = 0; t < NK; t++)
+                    if (t == j) {
+                        x = u[t];
+                        vl = vls[t / 3];
+                    }
+                const int r = hash_insert(key + vl * SLOTS, x);
+#pragma unroll
+                for (int t = 0; t < NK; t++)
+                    if (t == j) sl[t] = r;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < BATCH; k++) {
+            if (w[k][0] == 0xFFFFFFFFu) continue;
+            const int vl = vls[k];
             atomicAdd(&ccount[vl], 1);
-            const int sa = hash_insert(tab, &occ[vl], (int32_t)x);
-            const int sb = hash_insert(tab, &occ[vl], (int32_t)y);
-            const int sc = hash_insert(tab, &occ[vl], (int32_t)z);
-            if ((sa | sb | sc) < 0) {
+            const int a = sl[3 * k], b2 = sl[3 * k + 1], c = sl[3 * k + 2];
+            if ((a | b2 | c) < 0) {
                 ovf[vl] = 1;
             } else {
                 uint32_t *av = adj + vl * SLOTS;
-                atomicOr(&av[sa], (1u << sb) | (1u << sc));
-                atomicOr(&av[sb], (1u << sa) | (1u << sc));
-                atomicOr(&av[sc], (1u << sa) | (1u << sb));
+                atomicOr(&av[a], (1u << b2) | (1u << c));
+                atomicOr(&av[b2], (1u << a) | (1u << c));
+                atomicOr(&av[c], (1u << a) | (1u << b2));
             }
         }
+    }
+    __syncthreads();
+    // occupancy masks from the final tables
+    for (int vl = tid / SLOTS; vl < BV; vl += NT / SLOTS) {
+        const uint64_t bal = __ballot(key[vl * SLOTS + (tid & (SLOTS - 1))] >= 0);
+        if ((tid & (SLOTS - 1)) == 0) occ[vl] = (uint32_t)(bal >> (tid & 32));
     }
     __syncthreads();
 
@@ -441,15 +569,21 @@ __global__ __launch_bounds__(2 << BV_LOG) void k_bkt_vv_cp(
     if (ok) {
         const uint32_t om = occ[vl];
         d = __popc(om);
+        uint32_t lm = 0, pl[5] = {0, 0, 0, 0, 0};
         if (classify) {
-            const uint32_t lm = lowm[vl];
-            nl = n_components(adj + vl * SLOTS, om & lm);
-            nu = n_components(adj + vl * SLOTS, om & ~lm);
+            lm = lowm[vl] & om;
+            nl = label_components(adj + vl * SLOTS, lm, pl);
+            nu = label_components(adj + vl * SLOTS, om & ~lm, pl);
         }
+        // sort key: neighbour id, then (lower bit, component number) as the
+        // row descriptor the row-membership link pass reads
 #pragma unroll
         for (int i = 0; i < 32; i++) {
             const int32_t k = key[vl * SLOTS + i];
-            r[i] = k < 0 ? INT32_MAX : k;
+            const uint32_t desc = (((lm >> i) & 1u) << 5) | ((pl[0] >> i) & 1u) |
+                                  (((pl[1] >> i) & 1u) << 1) | (((pl[2] >> i) & 1u) << 2) |
+                                  (((pl[3] >> i) & 1u) << 3) | (((pl[4] >> i) & 1u) << 4);
+            r[i] = k < 0 ? INT32_MAX : (int32_t)(((uint32_t)k << 6) | desc);
         }
     }
     const int32_t hres = hv ? 3 * ccount[vl] : 0;
@@ -489,16 +623,21 @@ __global__ __launch_bounds__(2 << BV_LOG) void k_bkt_vv_cp(
     }
     if (act) rowsrc[v0 + vl] = 3 * cbeg + loff;
     wave_append(heavy, counters + C_HEAVY, v0 + vl, hv);
-    wave_append(fix, counters + C_FIX, v0 + vl, ok && classify && (nl >= 2 || nu >= 2));
+    wave_append(fixd, counters + tg::C_FIXD, v0 + vl, ok && classify && (nl >= 2 || nu >= 2));
     int dm = ok ? d : 0;
 #pragma unroll
     for (int s = 32; s >= 1; s >>= 1) dm = max(dm, __shfl_xor(dm, s, 64));
     if (lane == 0 && dm > 0) atomicMax(counters + C_MAXDEG, dm);
     __syncthreads();
 
-    // phase 4: coalesced store of the bucket's rows
+    // phase 4: coalesced store of the bucket's rows and row descriptors
     int32_t *dst = scratch + 3 * (int64_t)cbeg;
-    for (int i = tid; i < xt; i += NT) dst[i] = key[i];
+    uint8_t *ddst = rowdesc + 3 * (int64_t)cbeg;
+    for (int i = tid; i < xt; i += NT) {
+        const int32_t pk = key[i];
+        dst[i] = pk >> 6;
+        if (classify) ddst[i] = (uint8_t)(pk & 63);
+    }
 }
 
 // Serial fallback for vertices with more than 32 neighbours: lane 0 scans
@@ -642,12 +781,12 @@ int launch_bucketed_typed(tg::BucketBuffers &k, bool classify, hipStream_t s) {
     if (mk) mk->mark("l2_scatter", s);
     if (p.bv_log == 7)
         hipLaunchKernelGGL((k_bkt_vv_cp<7, S>), dim3(p.nb), dim3(256), 0, s, r2, k.h2, p.slices,
-                           scal, V, cls, b.scratch, b.deg, k.rowsrc, b.classes, b.heavy, b.fix,
-                           b.counters);
+                           scal, V, cls, b.scratch, b.deg, k.rowsrc, b.classes, b.heavy, b.fixd,
+                           b.rowdesc, b.counters);
     else
         hipLaunchKernelGGL((k_bkt_vv_cp<8, S>), dim3(p.nb), dim3(512), 0, s, r2, k.h2, p.slices,
-                           scal, V, cls, b.scratch, b.deg, k.rowsrc, b.classes, b.heavy, b.fix,
-                           b.counters);
+                           scal, V, cls, b.scratch, b.deg, k.rowsrc, b.classes, b.heavy, b.fixd,
+                           b.rowdesc, b.counters);
     TG_LAUNCH_CHECK("k_bkt_vv_cp");
     if (mk) mk->mark("bkt_vv_cp", s);
     if (p.bv_log == 7)
