@@ -142,6 +142,19 @@ int tg_profile_enable(tg_mesh *m, int on);
 int tg_profile_read(tg_mesh *m, int32_t max_stages, int32_t *n_stages,
                     float *ms, char *names);
 
+/* A built relation converted into new device buffers that the caller owns
+ * and releases with tg_device_free: values[nnz*width] as int32 or int64
+ * (value_bytes 4 or 8) and, for CSR relations when d_offsets is not NULL,
+ * offsets[rows+1] as uint32 or int64 (offset_bytes 4 or 8; NULL for fixed
+ * relations).  Backs the vtkIdType* / device_VV / device_VT results of
+ * make_X_GPU_return (include/cuda_extraction.h:27-138).  Synchronises. */
+int tg_relation_export(tg_mesh *m, int32_t rel, int32_t offset_bytes, void **d_offsets,
+                       int32_t value_bytes, void **d_values);
+/* Copy `bytes` of host memory into a new device buffer owned by the caller
+ * (released with tg_device_free).  Backs make_VT_GPU_return(const VT_Data&)
+ * (include/cuda_extraction.h:138), which uploads a host relation. */
+int tg_device_upload(const void *host, size_t bytes, void **d_out);
+
 /* Free a device pointer handed out by the library to the caller. */
 int tg_device_free(void *p);
 

@@ -1,0 +1,72 @@
+"""The reference's driver and GPU builders through the C++ API on the GPU:
+tetra_main builds every flagged relation with make_X_GPU (over the C ABI),
+rebuilds them with the host builders and checks them with the reference's
+tolerances and exactly, classifies critical points and exports the totals
+(src/alg/critPoints.cu:1359-1514).  The host builders themselves are checked
+against the oracle in test_dropin_cpu."""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import binding as O
+
+pytestmark = pytest.mark.gpu
+
+ROOT = Path(__file__).resolve().parent.parent
+MAIN = ROOT / "tetragpu_amd" / "lib" / "tetra_main"
+ALL_FLAGS = ["--build_" + r for r in ("TE", "EV", "ET", "TF", "FV", "FE", "FT", "EF", "VT", "TT",
+                                      "FF", "EE", "VV")]
+
+
+def tetra_main(*args, timeout=240):
+    print("tetra_main", *args, flush=True)
+    return subprocess.run([str(MAIN), *map(str, args)], capture_output=True, text=True,
+                          timeout=timeout)
+
+
+def totals(text):
+    out = {}
+    for line in text.splitlines():
+        if line.startswith("Total number of"):
+            k, v = line.split(":")
+            out[k.split()[-1]] = int(v)
+    return out
+
+
+@pytest.mark.parametrize("spec", ["kuhn:6:3:gaussians", "kuhn:5:11:uniform", "kuhn:4:2:constant"])
+def test_all_relations_validated(gpu, spec, tmp_path):
+    out = tmp_path / "classes.txt"
+    r = tetra_main("-i", spec, "-e", out, "-d", "1", *ALL_FLAGS)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "[validate] all checks passed" in r.stderr
+    assert "FAILED" not in r.stderr
+    # totals agree with the oracle on the same generated mesh
+    import tetragpu_amd as t
+    _, n, seed, scalar = spec.split(":")
+    m = t.kuhn(int(n), seed=int(seed), scalar=scalar)
+    cls = O.critpoints(O.vv(m.cells, m.n_points), m.scalars.astype(np.float64), m.n_points)
+    tot = totals(out.read_text())
+    assert tot["minima:"] == (cls[:, 2] == 2).sum()   # the reference's label for class 2
+    assert tot["maxima:"] == (cls[:, 2] == 1).sum()
+    assert tot["saddles:"] == (cls[:, 2] == 4).sum()
+    assert tot["regular:"] == (cls[:, 2] == 3).sum()
+
+
+def test_vtu_input_default_flags(gpu, tmp_path):
+    from test_dropin_cpu import write_vtu
+    import tetragpu_amd as t
+
+    m = t.kuhn(5, seed=8, scalar="sines")
+    path = tmp_path / "m.vtu"
+    write_vtu(path, m.cells, np.zeros((m.n_points, 3)), [("f", m.scalars)], "appended", True)
+    r = tetra_main("-i", path, "-d", "0")   # defaults: VV + validate, classes to stdout
+    assert r.returncode == 0, r.stderr[-2000:]
+    cls = O.critpoints(O.vv(m.cells, m.n_points), m.scalars.astype(np.float64), m.n_points)
+    assert totals(r.stdout)["saddles:"] == (cls[:, 2] == 4).sum()
+
+
+def test_bad_input_exits_with_failure(gpu, tmp_path):
+    r = tetra_main("-i", tmp_path / "missing.vtu")
+    assert r.returncode == 1 and "cannot open" in r.stderr

@@ -52,6 +52,8 @@ _HIP_SIGS = {
     "tg_labels6_copy": (C.c_int, [P, P]),
     "tg_approx_max_vv": (C.c_int, [P, C.POINTER(I32)]),
     "tg_device_free": (C.c_int, [P]),
+    "tg_relation_export": (C.c_int, [P, I32, I32, C.POINTER(P), I32, C.POINTER(P)]),
+    "tg_device_upload": (C.c_int, [P, C.c_size_t, C.POINTER(P)]),
     "tg_profile_enable": (C.c_int, [P, C.c_int]),
     "tg_profile_read": (C.c_int, [P, I32, C.POINTER(I32), P, P]),
 }

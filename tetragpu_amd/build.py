@@ -85,7 +85,7 @@ def build_host(jobs: int = 8) -> Path:
     so = LIB / "libtetra_host.so"
     tmp = LIB / "libtetra_host.so.tmp"
     _run([CXX, "-shared", "-fopenmp", *map(str, objs), "-o", str(tmp),
-          f"-L{LIB}", "-ltetra_hip", f"-Wl,-rpath,$ORIGIN"])
+          f"-L{LIB}", "-ltetra_hip", "-lz", f"-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, so)
     main_src = HOST_SRC / "tetra_main.cpp"
     if main_src.exists():

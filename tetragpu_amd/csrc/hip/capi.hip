@@ -29,7 +29,7 @@ int DevArr::ensure(size_t bytes) {
     if (bytes == 0) bytes = 256;
     if (p && cap >= bytes) return OK;
     if (p) {
-        hipFree(p);
+        (void)hipFree(p);
         p = nullptr;
         cap = 0;
     }
@@ -48,7 +48,7 @@ int DevArr::ensure(size_t bytes) {
 }
 
 void DevArr::release() {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
     p = nullptr;
     cap = 0;
 }
@@ -56,6 +56,13 @@ void DevArr::release() {
 }  // namespace tg
 
 namespace {
+
+template <typename A, typename B>
+__global__ void k_convert(const A *__restrict__ in, int64_t n, B *__restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (B)in[i];
+}
 
 __global__ void k_validate(const int4 *__restrict__ tv4, int64_t T, int64_t V,
                            int32_t *__restrict__ err) {
@@ -201,7 +208,7 @@ int tg_mesh_destroy(tg_mesh *m) {
     if (!m) return TG_OK;
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     m->release_all();
-    if (m->h_pinned) hipHostFree(m->h_pinned);
+    if (m->h_pinned) (void)hipHostFree(m->h_pinned);
     if (m->own_stream && m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
     return TG_OK;
@@ -415,6 +422,77 @@ int tg_profile_read(tg_mesh *m, int32_t max_stages, int32_t *n_stages, float *ms
             names[32 * i + 31] = 0;
         }
     }
+    return TG_OK;
+}
+
+int tg_relation_export(tg_mesh *m, int32_t rel, int32_t offset_bytes, void **d_offsets,
+                       int32_t value_bytes, void **d_values) {
+    if (!m || !d_values) return fail(TG_ERR_ARG, "NULL mesh or output");
+    if ((offset_bytes != 4 && offset_bytes != 8) || (value_bytes != 4 && value_bytes != 8))
+        return fail(TG_ERR_ARG, "offset_bytes and value_bytes must be 4 or 8");
+    *d_values = nullptr;
+    if (d_offsets) *d_offsets = nullptr;
+    tg::RelView r;
+    int rc = tg::relation_view(*m, rel, r);
+    if (rc) return rc;
+    const int64_t nv = r.nnz * r.width;
+    void *vals = nullptr, *offs = nullptr;
+    auto release = [&](int code) {
+        if (vals) (void)hipFree(vals);
+        if (offs) (void)hipFree(offs);
+        return code;
+    };
+    if (hipMalloc(&vals, (size_t)value_bytes * (size_t)std::max<int64_t>(nv, 1)) != hipSuccess)
+        return release(fail(TG_ERR_ALLOC, "hipMalloc of relation values failed"));
+    if (nv > 0) {
+        const unsigned g = std::min<unsigned>(tg::grid_for(nv, 256), 65536u);
+        if (value_bytes == 8)
+            hipLaunchKernelGGL((k_convert<int32_t, int64_t>), dim3(g), dim3(256), 0, m->stream,
+                               r.values, nv, (int64_t *)vals);
+        else
+            hipLaunchKernelGGL((k_convert<int32_t, int32_t>), dim3(g), dim3(256), 0, m->stream,
+                               r.values, nv, (int32_t *)vals);
+        if (hipGetLastError() != hipSuccess) return release(fail(TG_ERR_HIP, "export launch"));
+    }
+    if (d_offsets && r.offsets) {
+        const int64_t n = r.rows + 1;
+        if (hipMalloc(&offs, (size_t)offset_bytes * (size_t)n) != hipSuccess)
+            return release(fail(TG_ERR_ALLOC, "hipMalloc of relation offsets failed"));
+        const unsigned g = std::min<unsigned>(tg::grid_for(n, 256), 65536u);
+        if (r.offset_bytes == 8 && offset_bytes == 8)
+            hipLaunchKernelGGL((k_convert<int64_t, int64_t>), dim3(g), dim3(256), 0, m->stream,
+                               (const int64_t *)r.offsets, n, (int64_t *)offs);
+        else if (r.offset_bytes == 8)
+            hipLaunchKernelGGL((k_convert<int64_t, uint32_t>), dim3(g), dim3(256), 0, m->stream,
+                               (const int64_t *)r.offsets, n, (uint32_t *)offs);
+        else if (offset_bytes == 8)
+            hipLaunchKernelGGL((k_convert<int32_t, int64_t>), dim3(g), dim3(256), 0, m->stream,
+                               (const int32_t *)r.offsets, n, (int64_t *)offs);
+        else
+            hipLaunchKernelGGL((k_convert<int32_t, uint32_t>), dim3(g), dim3(256), 0, m->stream,
+                               (const int32_t *)r.offsets, n, (uint32_t *)offs);
+        if (hipGetLastError() != hipSuccess) return release(fail(TG_ERR_HIP, "export launch"));
+        if (r.offset_bytes == 8 && offset_bytes == 4 && r.nnz > (int64_t)UINT32_MAX)
+            return release(fail(TG_ERR_CAPACITY, "relation too large for 32-bit offsets"));
+    }
+    if (hipStreamSynchronize(m->stream) != hipSuccess)
+        return release(fail(TG_ERR_HIP, "export synchronisation failed"));
+    *d_values = vals;
+    if (d_offsets) *d_offsets = offs;
+    return TG_OK;
+}
+
+int tg_device_upload(const void *host, size_t bytes, void **d_out) {
+    if (!d_out || (bytes && !host)) return fail(TG_ERR_ARG, "NULL argument");
+    *d_out = nullptr;
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess)
+        return fail(TG_ERR_ALLOC, "hipMalloc failed");
+    if (bytes && hipMemcpy(p, host, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(p);
+        return fail(TG_ERR_HIP, "upload failed");
+    }
+    *d_out = p;
     return TG_OK;
 }
 

@@ -1,0 +1,391 @@
+// The reference's GPU relation builders (include/cuda_extraction.h:10-138)
+// over the C ABI of libtetra_hip.so.  The reference keeps TV/VE/VF in
+// file-scope device globals; here each calling thread has a current mesh
+// handle (created by make_TV_for_GPU, or by any builder that receives a TV
+// it has not seen), so worker threads do not share state.
+#include "cuda_extraction.h"
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "tetra_c.h"
+
+namespace {
+
+struct Session {
+    tg_mesh *mesh = nullptr;
+    const TV_Data *tv = nullptr;  // the TV the mesh was made from
+    vtkIdType n_points = 0, n_cells = 0;
+    bool initialised = false;
+};
+
+thread_local Session g_session;
+
+[[noreturn]] void die(const char *what, int status) {
+    std::fprintf(stderr, "tetra: %s failed [%d]: %s\n", what, status, tg_last_error());
+    std::exit(EXIT_FAILURE);
+}
+
+void ok(int status, const char *what) {
+    if (status != TG_OK) die(what, status);
+}
+
+void release_session() {
+    if (g_session.mesh) tg_mesh_destroy(g_session.mesh);
+    g_session.mesh = nullptr;
+    g_session.tv = nullptr;
+}
+
+// (Re)create the thread's mesh from TV: cells narrowed to int32 with a range
+// check, vertexAttributes (if any) as float64 scalars.
+void upload(const TV_Data &tv) {
+    if (!g_session.initialised) {
+        ok(tg_init(0), "tg_init");
+        g_session.initialised = true;
+    }
+    release_session();
+    if (tv.nPoints > INT32_MAX || 4 * tv.nCells > INT32_MAX) {
+        std::fprintf(stderr, "tetra: mesh too large for 32-bit device ids\n");
+        std::exit(EXIT_FAILURE);
+    }
+    std::vector<int32_t> cells((size_t)(4 * tv.nCells));
+    for (size_t i = 0; i < cells.size(); i++) {
+        const vtkIdType x = tv.cells[i];
+        if (x < 0 || x >= tv.nPoints) {
+            std::fprintf(stderr, "tetra: cell %zu references vertex %lld outside [0, %lld)\n",
+                         i / 4, x, tv.nPoints);
+            std::exit(EXIT_FAILURE);
+        }
+        cells[i] = (int32_t)x;
+    }
+    ok(tg_mesh_create(&g_session.mesh, cells.data(), tv.nCells, tv.nPoints, tv.vertexAttributes,
+                      1, 0, nullptr),
+       "tg_mesh_create");
+    g_session.tv = &tv;
+    g_session.n_points = tv.nPoints;
+    g_session.n_cells = tv.nCells;
+}
+
+tg_mesh *mesh_for(const TV_Data &tv) {
+    if (!g_session.mesh || g_session.tv != &tv || g_session.n_cells != tv.nCells ||
+        g_session.n_points != tv.nPoints)
+        upload(tv);
+    return g_session.mesh;
+}
+
+tg_mesh *current(const char *who) {
+    if (!g_session.mesh) {
+        std::fprintf(stderr, "tetra: %s needs make_TV_for_GPU first\n", who);
+        std::exit(EXIT_FAILURE);
+    }
+    return g_session.mesh;
+}
+
+void build(tg_mesh *m, int rel) { ok(tg_build(m, TG_FLAG(rel)), "tg_build"); }
+
+// host copy of a built relation: int64 offsets (empty for fixed) + values
+struct HostRel {
+    int64_t rows = 0, nnz = 0;
+    int32_t width = 1;
+    bool fixed = true;
+    std::vector<int64_t> off;
+    std::vector<int32_t> val;
+};
+
+HostRel fetch(tg_mesh *m, int rel) {
+    HostRel h;
+    int32_t fixed = 1;
+    ok(tg_relation_shape(m, rel, &h.rows, &h.nnz, &h.width, &fixed), "tg_relation_shape");
+    h.fixed = fixed != 0;
+    h.val.resize((size_t)(h.nnz * h.width));
+    if (!h.fixed) h.off.resize((size_t)(h.rows + 1));
+    ok(tg_relation_copy(m, rel, h.fixed ? nullptr : h.off.data(), h.val.data()),
+       "tg_relation_copy");
+    return h;
+}
+
+std::vector<std::vector<vtkIdType>> to_rows(const HostRel &h) {
+    std::vector<std::vector<vtkIdType>> out((size_t)h.rows);
+    for (int64_t r = 0; r < h.rows; r++)
+        out[r].assign(h.val.begin() + h.off[r], h.val.begin() + h.off[r + 1]);
+    return out;
+}
+
+template <size_t W>
+std::vector<std::array<vtkIdType, W>> to_fixed(const HostRel &h) {
+    std::vector<std::array<vtkIdType, W>> out((size_t)h.rows);
+    for (int64_t r = 0; r < h.rows; r++)
+        for (size_t k = 0; k < W; k++) out[r][k] = h.val[(size_t)r * W + k];
+    return out;
+}
+
+vtkIdType *export_values(tg_mesh *m, int rel) {
+    void *vals = nullptr;
+    ok(tg_relation_export(m, rel, 8, nullptr, 8, &vals), "tg_relation_export");
+    return (vtkIdType *)vals;
+}
+
+void expect_count(const char *what, vtkIdType got, vtkIdType want) {
+    if (got != want) {
+        std::fprintf(stderr, "tetra: %s: the device finds %lld, the caller's table has %lld\n",
+                     what, got, want);
+        std::exit(EXIT_FAILURE);
+    }
+}
+
+int64_t n_edges(tg_mesh *m) {
+    int64_t v, t, e, f;
+    build(m, TG_REL_EV);
+    ok(tg_mesh_counts(m, &v, &t, &e, &f), "tg_mesh_counts");
+    return e;
+}
+
+int64_t n_faces(tg_mesh *m) {
+    int64_t v, t, e, f;
+    build(m, TG_REL_FV);
+    ok(tg_mesh_counts(m, &v, &t, &e, &f), "tg_mesh_counts");
+    return f;
+}
+
+void finish(bool free_transients) {
+    if (free_transients) release_session();
+}
+
+int max_incidence_bound(const TV_Data &TV, const vtkIdType n_points, const vtkIdType *ivvi,
+                        int debug) {
+    std::vector<int> seen((size_t)n_points, 0);
+    int most = 0;
+    for (vtkIdType i = 0; i < 4 * TV.nCells; i++) {
+        const vtkIdType v = ivvi ? ivvi[TV.cells[i]] : TV.cells[i];
+        most = std::max(most, ++seen[v]);
+    }
+    const int bound = ((3 * most + 31) / 32) * 32;
+    if (debug > NO_DEBUG) std::fprintf(stderr, "Approximated max VV adjacency: %d\n", bound);
+    return bound;
+}
+
+}  // namespace
+
+void free_transients_GPU() { release_session(); }
+
+// The calling thread's current mesh handle (used by the critical-point
+// driver to run the C ABI on the mesh make_TV_for_GPU uploaded).
+tg_mesh *tetra_current_mesh() { return g_session.mesh; }
+
+void make_TV_for_GPU(const TV_Data &tv_relationship) { upload(tv_relationship); }
+
+void make_VE_for_GPU(const VE_Data &ve, const vtkIdType n_edges_, const vtkIdType n_verts) {
+    tg_mesh *m = current("make_VE_for_GPU");
+    expect_count("vertices of VE", (vtkIdType)ve.size(), n_verts);
+    expect_count("edges", n_edges(m), n_edges_);
+}
+
+void make_VF_for_GPU(const VF_Data &vf, const vtkIdType n_verts, const vtkIdType n_faces_) {
+    tg_mesh *m = current("make_VF_for_GPU");
+    expect_count("vertices of VF", (vtkIdType)vf.size(), n_verts);
+    expect_count("faces", n_faces(m), n_faces_);
+}
+
+vtkIdType *make_EV_GPU_return(const VE_Data &, const vtkIdType, const vtkIdType n_edges_,
+                              const bool free_transients) {
+    tg_mesh *m = current("make_EV_GPU_return");
+    expect_count("edges", n_edges(m), n_edges_);
+    vtkIdType *out = export_values(m, TG_REL_EV);
+    finish(free_transients);
+    return out;
+}
+
+std::unique_ptr<EV_Data> make_EV_GPU(const VE_Data &, const vtkIdType, const vtkIdType n_edges_,
+                                     const bool free_transients) {
+    tg_mesh *m = current("make_EV_GPU");
+    expect_count("edges", n_edges(m), n_edges_);
+    auto out = std::make_unique<EV_Data>(to_fixed<2>(fetch(m, TG_REL_EV)));
+    finish(free_transients);
+    return out;
+}
+
+vtkIdType *make_TF_GPU_return(const TV_Data &TV, const VF_Data &, const vtkIdType,
+                              const vtkIdType n_faces_, const vtkIdType,
+                              const bool free_transients) {
+    tg_mesh *m = mesh_for(TV);
+    expect_count("faces", n_faces(m), n_faces_);
+    build(m, TG_REL_TF);
+    vtkIdType *out = export_values(m, TG_REL_TF);
+    finish(free_transients);
+    return out;
+}
+
+std::unique_ptr<TF_Data> make_TF_GPU(const TV_Data &TV, const VF_Data &, const vtkIdType,
+                                     const vtkIdType n_faces_, const vtkIdType,
+                                     const bool free_transients) {
+    tg_mesh *m = mesh_for(TV);
+    expect_count("faces", n_faces(m), n_faces_);
+    build(m, TG_REL_TF);
+    auto out = std::make_unique<TF_Data>(to_fixed<4>(fetch(m, TG_REL_TF)));
+    finish(free_transients);
+    return out;
+}
+
+vtkIdType *make_TE_GPU_return(const TV_Data &TV, const VE_Data &, const vtkIdType,
+                              const vtkIdType n_edges_, const vtkIdType,
+                              const bool free_transients) {
+    tg_mesh *m = mesh_for(TV);
+    expect_count("edges", n_edges(m), n_edges_);
+    build(m, TG_REL_TE);
+    vtkIdType *out = export_values(m, TG_REL_TE);
+    finish(free_transients);
+    return out;
+}
+
+std::unique_ptr<TE_Data> make_TE_GPU(const TV_Data &TV, const VE_Data &, const vtkIdType,
+                                     const vtkIdType n_edges_, const vtkIdType,
+                                     const bool free_transients) {
+    tg_mesh *m = mesh_for(TV);
+    expect_count("edges", n_edges(m), n_edges_);
+    build(m, TG_REL_TE);
+    auto out = std::make_unique<TE_Data>(to_fixed<6>(fetch(m, TG_REL_TE)));
+    finish(free_transients);
+    return out;
+}
+
+vtkIdType *make_FV_GPU_return(const VF_Data &, const vtkIdType, const vtkIdType n_faces_,
+                              const bool free_transients) {
+    tg_mesh *m = current("make_FV_GPU_return");
+    expect_count("faces", n_faces(m), n_faces_);
+    vtkIdType *out = export_values(m, TG_REL_FV);
+    finish(free_transients);
+    return out;
+}
+
+std::unique_ptr<FV_Data> make_FV_GPU(const VF_Data &, const vtkIdType, const vtkIdType n_faces_,
+                                     const bool free_transients) {
+    tg_mesh *m = current("make_FV_GPU");
+    expect_count("faces", n_faces(m), n_faces_);
+    const HostRel h = fetch(m, TG_REL_FV);
+    auto out = std::make_unique<FV_Data>((size_t)h.rows);
+    for (int64_t f = 0; f < h.rows; f++)
+        (*out)[f] = FaceData(h.val[3 * f], h.val[3 * f + 1], h.val[3 * f + 2]);
+    finish(free_transients);
+    return out;
+}
+
+vtkIdType *make_FE_GPU_return(const VF_Data &, const VE_Data &, const vtkIdType,
+                              const vtkIdType n_edges_, const vtkIdType n_faces_,
+                              const bool free_transients) {
+    tg_mesh *m = current("make_FE_GPU_return");
+    expect_count("edges", n_edges(m), n_edges_);
+    expect_count("faces", n_faces(m), n_faces_);
+    build(m, TG_REL_FE);
+    vtkIdType *out = export_values(m, TG_REL_FE);
+    finish(free_transients);
+    return out;
+}
+
+std::unique_ptr<FE_Data> make_FE_GPU(const VF_Data &, const VE_Data &, const vtkIdType,
+                                     const vtkIdType n_edges_, const vtkIdType n_faces_,
+                                     const bool free_transients) {
+    tg_mesh *m = current("make_FE_GPU");
+    expect_count("edges", n_edges(m), n_edges_);
+    expect_count("faces", n_faces(m), n_faces_);
+    build(m, TG_REL_FE);
+    const HostRel h = fetch(m, TG_REL_FE);
+    auto out = std::make_unique<FE_Data>((size_t)h.rows);
+    for (int64_t f = 0; f < h.rows; f++)
+        for (int k = 0; k < 3; k++)
+            (*out)[f][k] = EdgeData(h.val[6 * f + 2 * k], h.val[6 * f + 2 * k + 1]);
+    finish(free_transients);
+    return out;
+}
+
+int get_approx_max_VV_partitioned(const TV_Data &TV, const vtkIdType n_points,
+                                  const vtkIdType *ivvi, const int debug) {
+    return max_incidence_bound(TV, n_points, ivvi, debug);
+}
+
+int get_approx_max_VV(const TV_Data &TV, const vtkIdType n_points, const int debug) {
+    return max_incidence_bound(TV, n_points, nullptr, debug);
+}
+
+device_VV *make_VV_GPU_return(const TV_Data &TV, const int, const int, const int,
+                              const bool free_transients) {
+    tg_mesh *m = mesh_for(TV);
+    build(m, TG_REL_VV);
+    void *off = nullptr, *val = nullptr;
+    ok(tg_relation_export(m, TG_REL_VV, 4, &off, 8, &val), "tg_relation_export");
+    finish(free_transients);
+    return new device_VV((vtkIdType *)val, (unsigned int *)off);
+}
+
+std::unique_ptr<VV_Data> make_VV_GPU(const TV_Data &TV, const int, const int,
+                                     const bool free_transients) {
+    tg_mesh *m = mesh_for(TV);
+    build(m, TG_REL_VV);
+    auto out = std::make_unique<VV_Data>(to_rows(fetch(m, TG_REL_VV)));
+    finish(free_transients);
+    return out;
+}
+
+device_VT *make_VT_GPU_return(const TV_Data &TV) {
+    tg_mesh *m = mesh_for(TV);
+    build(m, TG_REL_VT);
+    void *off = nullptr, *val = nullptr;
+    ok(tg_relation_export(m, TG_REL_VT, 8, &off, 8, &val), "tg_relation_export");
+    return new device_VT((vtkIdType *)val, (unsigned long long *)off);
+}
+
+device_VT *make_VT_GPU_return(const VT_Data &VT) {
+    std::vector<unsigned long long> off(VT.size() + 1, 0);
+    for (size_t v = 0; v < VT.size(); v++) off[v + 1] = off[v] + VT[v].size();
+    std::vector<vtkIdType> val;
+    val.reserve(off.back());
+    for (const auto &row : VT) val.insert(val.end(), row.begin(), row.end());
+    if (!g_session.initialised) {
+        ok(tg_init(0), "tg_init");
+        g_session.initialised = true;
+    }
+    void *d_off = nullptr, *d_val = nullptr;
+    ok(tg_device_upload(off.data(), off.size() * sizeof(off[0]), &d_off), "tg_device_upload");
+    ok(tg_device_upload(val.data(), val.size() * sizeof(val[0]), &d_val), "tg_device_upload");
+    return new device_VT((vtkIdType *)d_val, (unsigned long long *)d_off);
+}
+
+namespace {
+template <int REL>
+std::unique_ptr<std::vector<std::vector<vtkIdType>>> csr_relation(const TV_Data &TV,
+                                                                   bool free_transients) {
+    tg_mesh *m = mesh_for(TV);
+    build(m, REL);
+    auto out = std::make_unique<std::vector<std::vector<vtkIdType>>>(to_rows(fetch(m, REL)));
+    finish(free_transients);
+    return out;
+}
+}  // namespace
+
+std::unique_ptr<ET_Data> make_ET_GPU(const TV_Data &TV, const bool f) {
+    return csr_relation<TG_REL_ET>(TV, f);
+}
+std::unique_ptr<EF_Data> make_EF_GPU(const TV_Data &TV, const bool f) {
+    return csr_relation<TG_REL_EF>(TV, f);
+}
+std::unique_ptr<TT_Data> make_TT_GPU(const TV_Data &TV, const bool f) {
+    return csr_relation<TG_REL_TT>(TV, f);
+}
+std::unique_ptr<FF_Data> make_FF_GPU(const TV_Data &TV, const bool f) {
+    return csr_relation<TG_REL_FF>(TV, f);
+}
+std::unique_ptr<EE_Data> make_EE_GPU(const TV_Data &TV, const bool f) {
+    return csr_relation<TG_REL_EE>(TV, f);
+}
+std::unique_ptr<VT_Data> make_VT_GPU(const TV_Data &TV, const bool f) {
+    return csr_relation<TG_REL_VT>(TV, f);
+}
+
+std::unique_ptr<FT_Data> make_FT_GPU(const TV_Data &TV, const bool free_transients) {
+    tg_mesh *m = mesh_for(TV);
+    build(m, TG_REL_FT);
+    auto out = std::make_unique<FT_Data>(to_fixed<2>(fetch(m, TG_REL_FT)));
+    finish(free_transients);
+    return out;
+}
