@@ -28,6 +28,7 @@
 
 #include "cpu_extraction.h"
 #include "datatypes.h"
+#include "tetra_c.h"  // tg_device_free for the _return results
 
 void make_TV_for_GPU(const TV_Data &tv_relationship);
 void make_VE_for_GPU(const VE_Data &ve_relationship, const vtkIdType n_edges,

@@ -48,10 +48,10 @@ def test_all_relations_validated(gpu, spec, tmp_path):
     m = t.kuhn(int(n), seed=int(seed), scalar=scalar)
     cls = O.critpoints(O.vv(m.cells, m.n_points), m.scalars.astype(np.float64), m.n_points)
     tot = totals(out.read_text())
-    assert tot["minima:"] == (cls[:, 2] == 2).sum()   # the reference's label for class 2
-    assert tot["maxima:"] == (cls[:, 2] == 1).sum()
-    assert tot["saddles:"] == (cls[:, 2] == 4).sum()
-    assert tot["regular:"] == (cls[:, 2] == 3).sum()
+    assert tot["minima"] == (cls[:, 2] == 2).sum()   # the reference's label for class 2
+    assert tot["maxima"] == (cls[:, 2] == 1).sum()
+    assert tot["saddles"] == (cls[:, 2] == 4).sum()
+    assert tot["regular"] == (cls[:, 2] == 3).sum()
 
 
 def test_vtu_input_default_flags(gpu, tmp_path):
@@ -64,7 +64,7 @@ def test_vtu_input_default_flags(gpu, tmp_path):
     r = tetra_main("-i", path, "-d", "0")   # defaults: VV + validate, classes to stdout
     assert r.returncode == 0, r.stderr[-2000:]
     cls = O.critpoints(O.vv(m.cells, m.n_points), m.scalars.astype(np.float64), m.n_points)
-    assert totals(r.stdout)["saddles:"] == (cls[:, 2] == 4).sum()
+    assert totals(r.stdout)["saddles"] == (cls[:, 2] == 4).sum()
 
 
 def test_bad_input_exits_with_failure(gpu, tmp_path):
