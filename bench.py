@@ -65,6 +65,136 @@ def copy_calibration(torch, mib: int, reps: int) -> int:
     return 0
 
 
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def pct(xs, q):
+    import numpy as np
+    return round(float(np.percentile(np.asarray(xs), q)), 3) if len(xs) else None
+
+
+def gen_many(t, specs, threads=16):
+    """Generate meshes in parallel (the generator is native; ctypes drops the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(threads) as ex:
+        return list(ex.map(lambda kw: t.kuhn(**kw), specs))
+
+
+def bench_batch(t, torch, n_meshes: int, reps: int, seed: int = 7) -> dict:
+    """4096 Kuhn meshes, side log-uniform in [6, 40], uniform scalars: one
+    mesh per launch sequence vs the batch path (waves, hipGraph replays)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sides = np.floor(np.exp(rng.uniform(np.log(6), np.log(41), n_meshes))).astype(int)
+    t0 = time.time()
+    meshes = gen_many(t, [dict(n=int(n), seed=seed * 100003 + i, scalar="uniform")
+                          for i, n in enumerate(sides)])
+    gen_s = time.time() - t0
+    total_cells = int(sum(m.n_cells for m in meshes))
+    log(f"batch: {n_meshes} meshes, {total_cells} cells generated in {gen_s:.1f} s")
+    stream = torch.cuda.Stream()
+    # plain path: one resident mesh per request, one launch sequence each
+    singles = [t.Mesh(m.cells, m.n_points, m.scalars, stream=stream.cuda_stream) for m in meshes]
+    for d in singles:
+        d.vv_critpoints()
+    torch.cuda.synchronize()
+    loop_ms = []
+    for _ in range(reps):
+        a = time.perf_counter()
+        for d in singles:
+            d.vv_critpoints()
+        stream.synchronize()
+        loop_ms.append((time.perf_counter() - a) * 1e3)
+    loop_cls = np.concatenate([d.classes() for d in singles])
+    log(f"batch: one mesh per launch {min(loop_ms):.1f} ms")
+    for d in singles:
+        d.close()
+    # batch path
+    b = t.Batch([(m.cells, m.n_points, m.scalars) for m in meshes], stream=stream.cuda_stream)
+    b.use_graph(True)
+    b.vv_critpoints()  # eager warm-up (allocations)
+    b.vv_critpoints()  # capture + first replay
+    b.sync()
+    batch_ms = []
+    for _ in range(reps):
+        a = time.perf_counter()
+        b.vv_critpoints()
+        stream.synchronize()
+        batch_ms.append((time.perf_counter() - a) * 1e3)
+    same = bool(np.array_equal(b.classes(), loop_cls))
+    log(f"batch: batched {min(batch_ms):.1f} ms, identical={same}")
+    waves = b.counts()["waves"]
+    b.close()
+    lm, bm = min(loop_ms), min(batch_ms)
+    return {"meshes": n_meshes, "cells": total_cells, "waves": waves, "generate_s": round(gen_s, 1),
+            "loop_ms": round(lm, 3), "batch_ms": round(bm, 3),
+            "loop_meshes_per_s": round(n_meshes / (lm * 1e-3), 1),
+            "meshes_per_s": round(n_meshes / (bm * 1e-3), 1),
+            "speedup_vs_loop": round(lm / bm, 2), "tets_per_s": round(total_cells / (bm * 1e-3), 1),
+            "identical_to_loop": same, "graph": True}
+
+
+def bench_serve(t, rates, n_requests: int, seed: int = 11) -> dict:
+    """Poisson arrivals of VV + critical-point requests (Kuhn grids with 10 %
+    splits, T log-uniform in [10k, 5M], Gaussian-mixture scalars) followed by
+    a VT + TT query on the resident mesh; p50/p99 latency per rate."""
+    import threading
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+    from tetragpu_amd.serve import TT, VT, Server
+
+    rng = np.random.default_rng(seed)
+    pool_n = 24
+    targets = np.exp(rng.uniform(np.log(1e4), np.log(5e6), pool_n))
+    sides = np.maximum(6, np.round((targets / (6 * 1.3)) ** (1 / 3))).astype(int)
+    pool = gen_many(t, [dict(n=int(n), seed=seed * 7919 + i, split_fraction=0.1,
+                             scalar="gaussians") for i, n in enumerate(sides)], threads=8)
+    log(f"serve: pool of {pool_n} meshes generated")
+    out = {"pool_cells_min": int(min(m.n_cells for m in pool)),
+           "pool_cells_max": int(max(m.n_cells for m in pool)), "requests_per_rate": n_requests,
+           "rates": []}
+    for rate in rates:
+        lat1, lat2, cells = [], [], []
+        lock = threading.Lock()
+        with Server(slots=4, hbm_budget_bytes=96 << 30, fuse_max_cells=1 << 19,
+                    fuse_max_meshes=64) as srv, ThreadPoolExecutor(64) as ex:
+            def one(k, submitted):
+                m = pool[k]
+                rid = srv.submit(m.cells, m.n_points, m.scalars)
+                r1 = srv.wait(rid)
+                srv.followup(rid, VT | TT, copy=False)
+                done = time.perf_counter()
+                srv.release(rid)
+                with lock:
+                    lat1.append(r1["total_ms"])
+                    lat2.append((done - submitted) * 1e3)
+                    cells.append(m.n_cells)
+            picks = rng.integers(0, pool_n, n_requests)
+            gaps = rng.exponential(1.0 / rate, n_requests)
+            start = time.perf_counter()
+            nxt = start
+            futs = []
+            for k, g in zip(picks, gaps):
+                nxt += g
+                d = nxt - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                futs.append(ex.submit(one, int(k), time.perf_counter()))
+            for f in futs:
+                f.result()
+            wall = time.perf_counter() - start
+            groups = srv.stats()["groups"]
+        log(f"serve: rate {rate}: p50 {pct(lat1, 50)} ms p99 {pct(lat1, 99)} ms")
+        out["rates"].append({"rate": rate, "p50_ms": pct(lat1, 50), "p99_ms": pct(lat1, 99),
+                             "p50_ms_with_followup": pct(lat2, 50),
+                             "p99_ms_with_followup": pct(lat2, 99),
+                             "throughput_req_s": round(n_requests / wall, 1),
+                             "tets_per_s": round(sum(cells) / wall, 1), "launches": groups})
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +203,11 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=128, help="Kuhn grid side")
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--batch-meshes", type=int, default=4096,
+                    help="meshes in the batching measurement (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=60,
+                    help="requests per arrival rate in the serving measurement (0: skip)")
+    ap.add_argument("--rates", type=float, nargs="+", default=[100, 500, 1000, 2000, 5000])
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (reference kernel for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -91,6 +226,8 @@ def main() -> int:
         return copy_calibration(torch, args.copy_calibration, max(args.steps, 1))
     t.init(0)
     torch.cuda.init()
+    from tetragpu_amd import tuning
+    tuning.load()  # kernel form / shape per size class, measured on MI355X
     n = args.n
     workload = f"kuhn{n}_gaussians"
     t0 = time.time()
@@ -102,6 +239,7 @@ def main() -> int:
     stream = torch.cuda.Stream()
     d = t.Mesh(mesh.cells, V, mesh.scalars, stream=stream.cuda_stream)
 
+    log(f"workload {workload}: {T} tets, {V} vertices")
     for _ in range(max(args.warmup, 1)):
         d.vv_critpoints()
     d.sync()
@@ -153,6 +291,7 @@ def main() -> int:
         "stages_ms": {k: round(v, 4) for k, v in stages},
     }
 
+    log(f"{ms:.4f} ms per step")
     cpu = None
     if not args.no_cpu:
         from oracle import binding as O
@@ -164,6 +303,18 @@ def main() -> int:
                "sample": f"full workload ({T} tets), one pass of the OpenMP oracle "
                          f"(VV + critical points), {secs:.2f} s"}
     d.close()
+
+    # end to end from host buffers: upload TV + scalars, run, copy VV + classes back
+    t0 = time.perf_counter()
+    d2 = t.Mesh(mesh.cells, V, mesh.scalars, stream=stream.cuda_stream)
+    d2.vv_critpoints()
+    d2.relation("VV")
+    d2.classes()
+    e2e_ms = (time.perf_counter() - t0) * 1e3
+    d2.close()
+
+    batch = bench_batch(t, torch, args.batch_meshes, 3) if args.batch_meshes > 0 else None
+    serve = bench_serve(t, args.rates, args.serve_requests) if args.serve_requests > 0 else None
 
     line = {
         "metric": METRIC,
@@ -183,6 +334,10 @@ def main() -> int:
                    "generate_s": round(gen_s, 2)},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "end_to_end_ms": round(e2e_ms, 2),
+        "end_to_end_note": "host cells/scalars in, VV CSR + classes out, incl. allocation (first use)",
+        "batched": batch,
+        "serving": serve,
     }
     print(json.dumps(line))
     return 0

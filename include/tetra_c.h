@@ -96,6 +96,26 @@ int tg_mesh_counts(tg_mesh *m, int64_t *n_points, int64_t *n_cells,
 int tg_mesh_set_config(tg_mesh *m, int32_t variant, int32_t lane_cap,
                        int32_t lane_block);
 
+/* Shape of the bucketed form: 2^bv_log vertices per bucket (7 or 8) and
+ * tets per first-level routing chunk (>= 1024).  Results do not depend on
+ * it.  Chosen by the autotuner per size class. */
+int tg_mesh_set_bucket(tg_mesh *m, int32_t bv_log, int64_t tets_per_chunk);
+
+/* Process-wide tuning table used by variant 0 (automatic): the first entry
+ * (ascending max_cells) with n_cells <= max_cells gives the kernel form and
+ * its shape.  Written by the autotuner (include/tetra/autotune.h); no
+ * counterpart in the reference, which has fixed launch shapes. */
+typedef struct tg_tune_entry {
+    int64_t max_cells;
+    int32_t variant;     /* 1 vertex-lane, 2 bucketed */
+    int32_t lane_cap;    /* 32 or 64 */
+    int32_t lane_block;  /* 64, 128 or 256 */
+    int32_t bv_log;      /* 7 or 8 */
+    int64_t tets_per_chunk;
+} tg_tune_entry;
+int tg_set_tuning(const tg_tune_entry *entries, int32_t n);
+int tg_get_tuning(tg_tune_entry *entries, int32_t capacity, int32_t *n);
+
 /* Build the relations whose TG_FLAG bits are set, plus what they need.
  * Bit 0 (validate) is ignored.  Asynchronous. */
 int tg_build(tg_mesh *m, uint32_t flags);
@@ -154,6 +174,53 @@ int tg_relation_export(tg_mesh *m, int32_t rel, int32_t offset_bytes, void **d_o
  * (released with tg_device_free).  Backs make_VT_GPU_return(const VT_Data&)
  * (include/cuda_extraction.h:138), which uploads a host relation. */
 int tg_device_upload(const void *host, size_t bytes, void **d_out);
+
+/* ---- batches of independent meshes ----------------------------------------
+ * No counterpart in the reference (one mesh per process).  n_meshes meshes
+ * given as concatenated int32 cells with LOCAL vertex ids, cell_offsets and
+ * point_offsets [n_meshes + 1] (prefix sums, starting at 0), and optional
+ * concatenated scalars.  Meshes are grouped into waves of at most
+ * max_wave_cells cells (<= 0: the default); each wave runs the VV +
+ * critical-point path as one launch sequence over the union of its meshes
+ * (results are per mesh: rows and classes are vertex-local).  With
+ * tg_batch_use_graph(b, 1) each wave's sequence is captured into a
+ * hipGraph on its second run and replayed after.  A cell naming a vertex
+ * outside its own mesh is rejected (TG_ERR_MESH). */
+typedef struct tg_batch tg_batch;
+int tg_batch_create(tg_batch **out, int32_t n_meshes, const int64_t *cell_offsets,
+                    const int32_t *cells, const int64_t *point_offsets,
+                    const void *scalars, int scalar_f64, int64_t max_wave_cells,
+                    void *stream);
+int tg_batch_destroy(tg_batch *b);
+int tg_batch_counts(tg_batch *b, int32_t *n_waves, int64_t *n_cells,
+                    int64_t *n_points);
+int tg_batch_use_graph(tg_batch *b, int on);
+/* Kernel form for every wave (see tg_mesh_set_config / tg_mesh_set_bucket;
+ * bv_log and tets_per_chunk apply to variant 2). Drops captured graphs. */
+int tg_batch_set_config(tg_batch *b, int32_t variant, int32_t lane_cap,
+                        int32_t lane_block, int32_t bv_log,
+                        int64_t tets_per_chunk);
+/* VV + classes of every mesh. Asynchronous (graph replay when enabled). */
+int tg_batch_vv_critpoints(tg_batch *b);
+int tg_batch_sync(tg_batch *b);
+/* classes of all meshes, concatenated in mesh order: uint32 [sum V][3]. */
+int tg_batch_classes_copy(tg_batch *b, uint32_t *classes3);
+/* VV of one mesh with local ids: offsets [V_mesh + 1] (from 0) and values;
+ * either may be NULL; *nnz receives the entry count. Synchronises. */
+int tg_batch_vv_copy(tg_batch *b, int32_t mesh, int64_t *offsets, int32_t *values,
+                     int64_t *nnz);
+
+/* Follow-up relations on the resident batch: VV, VT and/or TT (TG_FLAG
+ * bits) of every wave, then one mesh's rows with local ids (VT/TT values
+ * are the mesh's own cell ids).  Copy synchronises. */
+int tg_batch_build(tg_batch *b, uint32_t flags);
+int tg_batch_relation_copy(tg_batch *b, int32_t mesh, int32_t rel, int64_t *offsets,
+                           int32_t *values, int64_t *nnz);
+
+/* A HIP stream at the default priority, for callers without a HIP runtime
+ * of their own (the serving loop's slots). */
+int tg_stream_create(void **stream);
+int tg_stream_destroy(void *stream);
 
 /* Free a device pointer handed out by the library to the caller. */
 int tg_device_free(void *p);

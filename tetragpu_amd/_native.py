@@ -54,6 +54,22 @@ _HIP_SIGS = {
     "tg_device_free": (C.c_int, [P]),
     "tg_relation_export": (C.c_int, [P, I32, I32, C.POINTER(P), I32, C.POINTER(P)]),
     "tg_device_upload": (C.c_int, [P, C.c_size_t, C.POINTER(P)]),
+    "tg_mesh_set_bucket": (C.c_int, [P, I32, I64]),
+    "tg_set_tuning": (C.c_int, [P, I32]),
+    "tg_get_tuning": (C.c_int, [P, I32, C.POINTER(I32)]),
+    "tg_batch_create": (C.c_int, [C.POINTER(P), I32, P, P, P, P, C.c_int, I64, P]),
+    "tg_batch_destroy": (C.c_int, [P]),
+    "tg_batch_counts": (C.c_int, [P, C.POINTER(I32), C.POINTER(I64), C.POINTER(I64)]),
+    "tg_batch_use_graph": (C.c_int, [P, C.c_int]),
+    "tg_batch_set_config": (C.c_int, [P, I32, I32, I32, I32, I64]),
+    "tg_batch_vv_critpoints": (C.c_int, [P]),
+    "tg_batch_sync": (C.c_int, [P]),
+    "tg_batch_classes_copy": (C.c_int, [P, P]),
+    "tg_batch_vv_copy": (C.c_int, [P, I32, P, P, C.POINTER(I64)]),
+    "tg_batch_build": (C.c_int, [P, U32]),
+    "tg_stream_create": (C.c_int, [C.POINTER(P)]),
+    "tg_stream_destroy": (C.c_int, [P]),
+    "tg_batch_relation_copy": (C.c_int, [P, I32, I32, P, P, C.POINTER(I64)]),
     "tg_profile_enable": (C.c_int, [P, C.c_int]),
     "tg_profile_read": (C.c_int, [P, I32, C.POINTER(I32), P, P]),
 }
@@ -62,6 +78,9 @@ _HOST_SIGS = {
     "tg_gen_kuhn": (C.c_int, [I32, C.c_uint64, U32, C.c_double, I32, I32, I32, P]),
     "tg_gen_scalars": (C.c_int, [P, I32, C.c_uint64]),
     "tg_gen_free": (None, [P]),
+    "tg_autotune": (C.c_int, [P, I32, I32, P, I32, C.POINTER(I32)]),
+    "tg_tuning_load": (C.c_int, [C.c_char_p]),
+    "tg_tuning_save": (C.c_int, [C.c_char_p]),
 }
 
 

@@ -29,7 +29,7 @@ struct BucketBuffers {
     int32_t *rowsrc = nullptr;  // [V] row start in scratch
 };
 
-BucketPlan plan_buckets(int64_t n_cells, int64_t n_points, int bv_log = 7);
+BucketPlan plan_buckets(int64_t n_cells, int64_t n_points, int bv_log = 7, int64_t tpc = 8192);
 int launch_bucketed_vv_cp(BucketBuffers &k, bool classify, hipStream_t s);
 // Whole bucketed pipeline: counters, routing, VV rows, CSR, link pass.
 int run_vv_cp_bucketed(BucketBuffers &k, bool classify, hipStream_t s);

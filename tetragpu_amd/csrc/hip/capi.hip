@@ -246,7 +246,31 @@ int tg_mesh_set_config(tg_mesh *m, int32_t variant, int32_t lane_cap, int32_t la
     m->cfg.variant = variant;
     m->cfg.lane_cap = lane_cap;
     m->cfg.lane_block = lane_block;
+    m->cfg.tuned = false;
+    m->hot_ready = false;
+    m->bk_ready = false;
     return TG_OK;
+}
+
+int tg_mesh_set_bucket(tg_mesh *m, int32_t bv_log, int64_t tets_per_chunk) {
+    if (!m) return fail(TG_ERR_ARG, "NULL mesh");
+    if (bv_log != 7 && bv_log != 8) return fail(TG_ERR_ARG, "bv_log must be 7 or 8");
+    if (tets_per_chunk < 1024) return fail(TG_ERR_ARG, "tets_per_chunk must be >= 1024");
+    m->cfg.bv_log = bv_log;
+    m->cfg.tpc = tets_per_chunk;
+    m->hot_ready = false;  // re-plan the buckets on the next run
+    m->bk_ready = false;
+    return TG_OK;
+}
+
+int tg_set_tuning(const tg_tune_entry *entries, int32_t n) {
+    if (n < 0 || (n > 0 && !entries)) return fail(TG_ERR_ARG, "bad tuning table");
+    return tg::set_tuning(entries, n);
+}
+
+int tg_get_tuning(tg_tune_entry *entries, int32_t capacity, int32_t *n) {
+    if (!n || (capacity > 0 && !entries)) return fail(TG_ERR_ARG, "NULL argument");
+    return tg::get_tuning(entries, capacity, n);
 }
 
 int tg_build(tg_mesh *m, uint32_t flags) {
@@ -493,6 +517,19 @@ int tg_device_upload(const void *host, size_t bytes, void **d_out) {
         return fail(TG_ERR_HIP, "upload failed");
     }
     *d_out = p;
+    return TG_OK;
+}
+
+int tg_stream_create(void **stream) {
+    if (!stream) return fail(TG_ERR_ARG, "NULL argument");
+    hipStream_t s = nullptr;
+    TG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = (void *)s;
+    return TG_OK;
+}
+
+int tg_stream_destroy(void *stream) {
+    if (stream) TG_HIP(hipStreamDestroy((hipStream_t)stream));
     return TG_OK;
 }
 

@@ -43,6 +43,9 @@ struct VvCpConfig {
     int lane_cap = 32;     // neighbours held per lane (32 or 64)
     int lane_block = 128;  // lanes (vertices) per workgroup
     int corner_block = 256;
+    int bv_log = 7;        // bucketed form: 2^bv_log vertices per bucket (7 or 8)
+    int64_t tpc = 8192;    // bucketed form: tets per first-level chunk
+    bool tuned = false;    // variant 0 resolved from the tuning table
 };
 
 // Device buffers of one VV + critical point run.  Sizes:

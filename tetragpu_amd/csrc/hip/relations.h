@@ -2,6 +2,7 @@
 #pragma once
 
 #include "bucket.h"
+#include "tetra_c.h"
 
 namespace tg {
 
@@ -74,6 +75,9 @@ namespace tg {
 int prepare_hot_path(tg_mesh &m);
 int prepare_vt(tg_mesh &m);
 int prepare_buckets(tg_mesh &m);
+// Process-wide tuning table (tg_set_tuning / tg_get_tuning).
+int set_tuning(const tg_tune_entry *e, int32_t n);
+int get_tuning(tg_tune_entry *e, int32_t cap, int32_t *n);
 // Variant the hot path runs for this mesh (1 vertex-lane, 2 bucketed).
 int hot_variant(const tg_mesh &m);
 // Enqueue VV + classes (recomputed from TV and scalars).

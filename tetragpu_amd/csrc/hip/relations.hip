@@ -1,7 +1,9 @@
 // Relation dispatch behind tg_build / tg_relation_*: buffer management of the
-// mesh handle, the VV + critical point hot path, VT, and (ids.hip,
-// derived.hip) the id-assigning and derived relations.
+// mesh handle, the VV + critical point hot path, VT, the tuning table, and
+// (in ids.hip) the id-assigning and derived relations.
 #include <algorithm>
+#include <mutex>
+#include <vector>
 #include <cstdio>
 
 #include "ids.h"
@@ -79,6 +81,8 @@ VvCpBuffers hot_buffers(tg_mesh &m) {
     return b;
 }
 
+void resolve_tuning(tg_mesh &m);
+
 int prepare_hot_path(tg_mesh &m) {
     if (m.hot_ready) return OK;
     const size_t V = (size_t)m.n_points, T = (size_t)m.n_cells, I = sizeof(int32_t);
@@ -90,9 +94,54 @@ int prepare_hot_path(tg_mesh &m) {
         (rc = m.counters.ensure(I * N_COUNTERS)) ||
         (rc = m.scan_tmp.ensure(scan_tmp_bytes(std::max<int64_t>(m.n_points, 12 * m.n_cells)))))
         return rc;
-    m.bplan = plan_buckets(m.n_cells, m.n_points);
+    resolve_tuning(m);
+    m.bplan = plan_buckets(m.n_cells, m.n_points, m.cfg.bv_log, m.cfg.tpc);
     m.hot_ready = true;
     return OK;
+}
+
+// ---- tuning table (process-wide, set by the autotuner) -----------------------------
+namespace {
+std::mutex g_tune_mu;
+std::vector<tg_tune_entry> g_tune;  // ascending max_cells
+}  // namespace
+
+int set_tuning(const tg_tune_entry *e, int32_t n) {
+    std::vector<tg_tune_entry> t(e, e + n);
+    for (const auto &x : t)
+        if ((x.variant != 1 && x.variant != 2) || (x.lane_cap != 32 && x.lane_cap != 64) ||
+            (x.lane_block != 64 && x.lane_block != 128 && x.lane_block != 256) ||
+            (x.lane_cap == 64 && x.lane_block == 256) || (x.bv_log != 7 && x.bv_log != 8) ||
+            x.tets_per_chunk < 1024)
+            return fail(ERR_ARG, "invalid tuning entry");
+    std::sort(t.begin(), t.end(),
+              [](const tg_tune_entry &a, const tg_tune_entry &b) { return a.max_cells < b.max_cells; });
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune = std::move(t);
+    return OK;
+}
+
+int get_tuning(tg_tune_entry *e, int32_t cap, int32_t *n) {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    *n = (int32_t)g_tune.size();
+    for (int32_t i = 0; i < cap && i < *n; i++) e[i] = g_tune[i];
+    return OK;
+}
+
+// variant 0: the first table entry whose max_cells covers the mesh
+void resolve_tuning(tg_mesh &m) {
+    if (m.cfg.variant != 0 && !m.cfg.tuned) return;
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    for (const auto &x : g_tune)
+        if (m.n_cells <= x.max_cells) {
+            m.cfg.variant = x.variant;
+            m.cfg.lane_cap = x.lane_cap;
+            m.cfg.lane_block = x.lane_block;
+            m.cfg.bv_log = x.bv_log;
+            m.cfg.tpc = x.tets_per_chunk;
+            m.cfg.tuned = true;
+            return;
+        }
 }
 
 int prepare_vt(tg_mesh &m) {

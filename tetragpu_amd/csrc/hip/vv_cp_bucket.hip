@@ -812,7 +812,7 @@ int ceil_log2(int64_t x) {
 
 namespace tg {
 
-BucketPlan plan_buckets(int64_t n_cells, int64_t n_points, int bv_log) {
+BucketPlan plan_buckets(int64_t n_cells, int64_t n_points, int bv_log, int64_t tpc) {
     BucketPlan p;
     const int64_t V = n_points, T = n_cells;
     p.bv_log = bv_log;
@@ -821,7 +821,7 @@ BucketPlan plan_buckets(int64_t n_cells, int64_t n_points, int bv_log) {
     p.fs = std::max(0, ceil_log2(nb) - 8);  // coarse bins <= 256
     p.nc = (int32_t)((nb + (1 << p.fs) - 1) >> p.fs);
     p.ok = V > 0 && V <= (int64_t)M24 + 1 && (1 << p.fs) <= MAX_FINE && p.nc <= MAX_COARSE;
-    p.tpc = 8192;
+    p.tpc = std::max<int64_t>(1024, tpc);
     p.nchunks = (int32_t)std::max<int64_t>(1, (T + p.tpc - 1) / p.tpc);
     p.slices = std::max(1, std::min(64, (1024 + p.nc - 1) / p.nc));
     return p;
