@@ -222,6 +222,18 @@ int tg_batch_relation_copy(tg_batch *b, int32_t mesh, int32_t rel, int64_t *offs
 int tg_stream_create(void **stream);
 int tg_stream_destroy(void *stream);
 
+/* The library keeps freed device buffers in a stream-ordered cache for
+ * reuse by later meshes (capped by TG_POOL_LIMIT_MB, default 64 GiB).
+ * tg_pool_trim returns every cached block to the runtime. */
+int tg_pool_trim(void);
+int tg_pool_cached(int64_t *bytes);
+
+/* Pinned host memory from a reusing cache (fast device->host copies of
+ * large relations without page-locking cost per call); TG_HOST_POOL_LIMIT_MB
+ * caps what stays cached (default 16 GiB). */
+int tg_host_alloc(size_t bytes, void **out);
+int tg_host_free(void *p);
+
 /* Free a device pointer handed out by the library to the caller. */
 int tg_device_free(void *p);
 

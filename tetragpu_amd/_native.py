@@ -68,6 +68,10 @@ _HIP_SIGS = {
     "tg_batch_vv_copy": (C.c_int, [P, I32, P, P, C.POINTER(I64)]),
     "tg_batch_build": (C.c_int, [P, U32]),
     "tg_stream_create": (C.c_int, [C.POINTER(P)]),
+    "tg_pool_trim": (C.c_int, []),
+    "tg_host_alloc": (C.c_int, [C.c_size_t, C.POINTER(P)]),
+    "tg_host_free": (C.c_int, [P]),
+    "tg_pool_cached": (C.c_int, [C.POINTER(I64)]),
     "tg_stream_destroy": (C.c_int, [P]),
     "tg_batch_relation_copy": (C.c_int, [P, I32, I32, P, P, C.POINTER(I64)]),
     "tg_profile_enable": (C.c_int, [P, C.c_int]),

@@ -78,11 +78,13 @@ int build_wave(tg_batch &b, Wave &w, const int32_t *cells, const void *scalars, 
     int32_t *d_cells = nullptr;
     int64_t *d_coff = nullptr, *d_poff = nullptr;
     int32_t *d_err = nullptr;
+    size_t g1 = 0, g2 = 0, g3 = 0, g4 = 0;
     auto release = [&](int code) {
-        if (d_cells) (void)hipFree(d_cells);
-        if (d_coff) (void)hipFree(d_coff);
-        if (d_poff) (void)hipFree(d_poff);
-        if (d_err) (void)hipFree(d_err);
+        // the stream has been synchronised (or nothing was enqueued) by now
+        tg::pool_free(d_cells, g1, b.stream);
+        tg::pool_free(d_coff, g2, b.stream);
+        tg::pool_free(d_poff, g3, b.stream);
+        tg::pool_free(d_err, g4, b.stream);
         return code;
     };
     std::vector<int64_t> coff(w.count + 1), poff(w.count + 1);
@@ -90,10 +92,11 @@ int build_wave(tg_batch &b, Wave &w, const int32_t *cells, const void *scalars, 
         coff[i] = b.coff[w.first + i] - w.cell0;
         poff[i] = b.poff[w.first + i] - w.point0;
     }
-    if (hipMalloc(&d_cells, sizeof(int32_t) * 4 * (size_t)std::max<int64_t>(nc, 1)) != hipSuccess ||
-        hipMalloc(&d_coff, sizeof(int64_t) * coff.size()) != hipSuccess ||
-        hipMalloc(&d_poff, sizeof(int64_t) * poff.size()) != hipSuccess ||
-        hipMalloc(&d_err, sizeof(int32_t)) != hipSuccess)
+    d_cells = (int32_t *)tg::pool_alloc(sizeof(int32_t) * 4 * (size_t)std::max<int64_t>(nc, 1), &g1);
+    d_coff = (int64_t *)tg::pool_alloc(sizeof(int64_t) * coff.size(), &g2);
+    d_poff = (int64_t *)tg::pool_alloc(sizeof(int64_t) * poff.size(), &g3);
+    d_err = (int32_t *)tg::pool_alloc(sizeof(int32_t), &g4);
+    if (!d_cells || !d_coff || !d_poff || !d_err)
         return release(fail(tg::ERR_ALLOC, "batch: device allocation failed"));
     hipError_t e = hipMemcpyAsync(d_cells, cells + 4 * w.cell0, sizeof(int32_t) * 4 * (size_t)nc,
                                   hipMemcpyHostToDevice, b.stream);

@@ -28,27 +28,21 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line) {
 int DevArr::ensure(size_t bytes) {
     if (bytes == 0) bytes = 256;
     if (p && cap >= bytes) return OK;
-    if (p) {
-        (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-    hipError_t e = hipMalloc(&p, bytes);
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        p = nullptr;
+    release();
+    size_t granted = 0;
+    p = pool_alloc(bytes, &granted);
+    if (!p) {
         char buf[160];
-        std::snprintf(buf, sizeof buf, "hipMalloc of %zu bytes failed: %s", bytes,
-                      hipGetErrorString(e));
+        std::snprintf(buf, sizeof buf, "device allocation of %zu bytes failed", bytes);
         set_error(buf);
         return ERR_ALLOC;
     }
-    cap = bytes;
+    cap = granted;
     return OK;
 }
 
 void DevArr::release() {
-    if (p) (void)hipFree(p);
+    if (p) pool_free(p, cap, stream);
     p = nullptr;
     cap = 0;
 }
@@ -169,14 +163,15 @@ int tg_mesh_create(tg_mesh **out, const int32_t *cells, int64_t n_cells,
         }
         m->own_stream = true;
     }
+    m->bind_stream(m->stream);
     auto bail = [&](int code) {
         tg_mesh_destroy(m);
         return code;
     };
     if ((rc = m->tv.ensure(sizeof(int32_t) * 4 * (size_t)n_cells))) return bail(rc);
     if ((rc = m->counters.ensure(sizeof(int32_t) * tg::N_COUNTERS))) return bail(rc);
-    if (hipHostMalloc(&m->h_pinned, 64 * sizeof(int64_t)) != hipSuccess)
-        return bail(fail(TG_ERR_ALLOC, "hipHostMalloc failed"));
+    if (!(m->h_pinned = tg::pinned_slot_alloc()))
+        return bail(fail(TG_ERR_ALLOC, "pinned staging allocation failed"));
     if (n_cells > 0) {
         hipError_t e = hipMemcpyAsync(m->tv.p, cells, sizeof(int32_t) * 4 * (size_t)n_cells,
                                       inputs_on_device ? hipMemcpyDeviceToDevice
@@ -208,7 +203,7 @@ int tg_mesh_destroy(tg_mesh *m) {
     if (!m) return TG_OK;
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     m->release_all();
-    if (m->h_pinned) (void)hipHostFree(m->h_pinned);
+    if (m->h_pinned) tg::pinned_slot_free(m->h_pinned);
     if (m->own_stream && m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
     return TG_OK;
@@ -517,6 +512,28 @@ int tg_device_upload(const void *host, size_t bytes, void **d_out) {
         return fail(TG_ERR_HIP, "upload failed");
     }
     *d_out = p;
+    return TG_OK;
+}
+
+int tg_host_alloc(size_t bytes, void **out) {
+    if (!out) return fail(TG_ERR_ARG, "NULL argument");
+    *out = tg::host_alloc(bytes);
+    return *out ? TG_OK : fail(TG_ERR_ALLOC, "pinned host allocation failed");
+}
+
+int tg_host_free(void *p) {
+    tg::host_free(p);
+    return TG_OK;
+}
+
+int tg_pool_trim(void) {
+    tg::pool_trim();
+    return TG_OK;
+}
+
+int tg_pool_cached(int64_t *bytes) {
+    if (!bytes) return fail(TG_ERR_ARG, "NULL argument");
+    *bytes = (int64_t)tg::pool_cached_bytes();
     return TG_OK;
 }
 

@@ -58,6 +58,20 @@ int scan_exclusive_i64(const int64_t *in, int64_t *out, int64_t n, void *tmp,
 int scan_exclusive_i32_to_i64(const int32_t *in, int64_t *out, int64_t n,
                               void *tmp, hipStream_t s);
 
+// Stream-ordered caching allocator (pool.hip): a freed block is reused only
+// after `stream` has passed the free.  pool_alloc returns nullptr on failure
+// and the usable size in *granted.
+void *pool_alloc(size_t bytes, size_t *granted);
+void pool_free(void *p, size_t bytes, hipStream_t stream);
+void pool_trim();
+size_t pool_cached_bytes();
+// Pinned host blocks from a size-classed cache (results handed to callers).
+void *host_alloc(size_t bytes);
+void host_free(void *p);
+// 512-byte pinned host slots for small device->host reads.
+void *pinned_slot_alloc();
+void pinned_slot_free(void *p);
+
 // Fill kernels (used instead of hipMemsetAsync so launch sequences can be
 // captured into graphs).
 int fill_i32(int32_t *p, int32_t value, int64_t n, hipStream_t s);

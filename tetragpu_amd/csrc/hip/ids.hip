@@ -204,16 +204,15 @@ __global__ void k_et(const int32_t *__restrict__ ev, int64_t E, int64_t V,
         int32_t i = ok ? vt_off[a] : 0, ie = ok ? vt_off[a + 1] : 0;
         int32_t j = ok ? vt_off[b] : 0, je = ok ? vt_off[b + 1] : 0;
         int64_t n = 0, o = FILL ? cnt_or_off[e] : 0;
-        while (i < ie && j < je) {
-            const int32_t x = vtc[i] >> 2, y = vtc[j] >> 2;
-            if (x < y) i++;
-            else if (y < x) j++;
-            else {
+        int32_t x = i < ie ? vtc[i] >> 2 : INF, y = j < je ? vtc[j] >> 2 : INF;
+        while (x != INF && y != INF) {
+            if (x == y) {
                 if (FILL) out[o + n] = x;
                 n++;
-                i++;
-                j++;
             }
+            const int32_t m = min(x, y);
+            if (x == m) x = ++i < ie ? vtc[i] >> 2 : INF;
+            if (y == m) y = ++j < je ? vtc[j] >> 2 : INF;
         }
         if (!FILL) cnt_or_off[e] = n;
     }
@@ -399,7 +398,9 @@ __global__ void k_ef(const int32_t *__restrict__ ev, int64_t E, const int64_t *_
     }
 }
 
-// TT(t): union of VT rows of t's corners, minus t
+// TT(t): union of VT rows of t's corners, minus t.  4-way merge of the
+// sorted rows with the four heads kept in registers: each step loads only
+// the heads it advances.
 template <bool FILL>
 __global__ void k_tt(const int4 *__restrict__ tv4, int64_t T, const int32_t *__restrict__ vt_off,
                      const int32_t *__restrict__ vtc, int64_t *__restrict__ cnt_or_off,
@@ -407,23 +408,21 @@ __global__ void k_tt(const int4 *__restrict__ tv4, int64_t T, const int32_t *__r
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < T;
          t += (int64_t)gridDim.x * blockDim.x) {
         const int4 q = tv4[t];
-        int32_t pos[4], end[4];
-        for (int j = 0; j < 4; j++) {
-            const int32_t v = corner(q, j);
-            pos[j] = vt_off[v];
-            end[j] = vt_off[v + 1];
-        }
-        int64_t n = 0, o = FILL ? cnt_or_off[t] : 0;
-        // 4-way merge with duplicate removal
+        int32_t p0 = vt_off[q.x], e0 = vt_off[q.x + 1], p1 = vt_off[q.y], e1 = vt_off[q.y + 1];
+        int32_t p2 = vt_off[q.z], e2 = vt_off[q.z + 1], p3 = vt_off[q.w], e3 = vt_off[q.w + 1];
+        int32_t h0 = p0 < e0 ? vtc[p0] >> 2 : INF, h1 = p1 < e1 ? vtc[p1] >> 2 : INF;
+        int32_t h2 = p2 < e2 ? vtc[p2] >> 2 : INF, h3 = p3 < e3 ? vtc[p3] >> 2 : INF;
+        int64_t n = 0;
+        int32_t *row = FILL ? out + cnt_or_off[t] : nullptr;
         while (true) {
-            int32_t mn = INF;
-            for (int j = 0; j < 4; j++)
-                if (pos[j] < end[j]) mn = min(mn, vtc[pos[j]] >> 2);
+            const int32_t mn = min(min(h0, h1), min(h2, h3));
             if (mn == INF) break;
-            for (int j = 0; j < 4; j++)
-                if (pos[j] < end[j] && (vtc[pos[j]] >> 2) == mn) pos[j]++;
+            if (h0 == mn) h0 = ++p0 < e0 ? vtc[p0] >> 2 : INF;
+            if (h1 == mn) h1 = ++p1 < e1 ? vtc[p1] >> 2 : INF;
+            if (h2 == mn) h2 = ++p2 < e2 ? vtc[p2] >> 2 : INF;
+            if (h3 == mn) h3 = ++p3 < e3 ? vtc[p3] >> 2 : INF;
             if (mn == (int32_t)t) continue;
-            if (FILL) out[o + n] = mn;
+            if (FILL) row[n] = mn;
             n++;
         }
         if (!FILL) cnt_or_off[t] = n;

@@ -10,6 +10,7 @@ namespace tg {
 struct DevArr {
     void *p = nullptr;
     size_t cap = 0;
+    hipStream_t stream = nullptr;  // the stream that uses it (orders its reuse)
     int ensure(size_t bytes);
     void release();
     template <typename T> T *as() const { return (T *)p; }
@@ -66,6 +67,7 @@ struct tg_mesh {
     tg::StageMarks marks;      // per-stage events of the last pipeline run
 
     void release_all();
+    void bind_stream(hipStream_t s);  // every DevArr of the mesh uses stream s
 };
 
 namespace tg {

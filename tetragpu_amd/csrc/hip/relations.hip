@@ -36,18 +36,31 @@ int read_i32(tg_mesh &m, const void *src, int64_t index, int32_t &out) {
 
 }  // namespace
 
-void tg_mesh::release_all() {
-    tg::DevArr *all[] = {&tv,       &scal,     &vt_cnt,   &vt_off,   &vt_cur,   &vtc,
-                         &scratch,  &deg,      &vv_off,   &vv_cols,  &classes,  &heavy,
-                         &fix,      &fixbig,   &counters, &scan_tmp, &labels,   &vt_vals,
-                         &edge_first, &ve_off, &ve_vals,  &ev,       &te,       &vv_eid,
-                         &et_off,   &et_vals,  &face_off, &vf_vals,  &fv,       &tf,
-                         &fe,       &ft,       &ef_off,   &ef_vals,  &tt_off,   &tt_vals,
-                         &ff_off,   &ff_vals,  &ee_off,   &ee_vals,  &bitmap,   &rank_tmp,
-                         &tmp_a,    &tmp_b,    &tmp_c,    &bk_h1,    &bk_h2,    &bk_rowsrc,
-                         &bk_fixd,  &bk_rowdesc};
-    for (auto *a : all) a->release();
-    for (auto &e : marks.ev)
+static void release_all_impl(tg_mesh *m, hipStream_t s, bool release);
+
+void tg_mesh::bind_stream(hipStream_t s) {
+    release_all_impl(this, s, false);
+}
+
+void tg_mesh::release_all() { release_all_impl(this, stream, true); }
+
+static void release_all_impl(tg_mesh *m, hipStream_t s, bool release) {
+    tg_mesh &me = *m;
+    tg::DevArr *all[] = {&me.tv,       &me.scal,     &me.vt_cnt,   &me.vt_off,   &me.vt_cur,   &me.vtc,
+                         &me.scratch,  &me.deg,      &me.vv_off,   &me.vv_cols,  &me.classes,  &me.heavy,
+                         &me.fix,      &me.fixbig,   &me.counters, &me.scan_tmp, &me.labels,   &me.vt_vals,
+                         &me.edge_first, &me.ve_off, &me.ve_vals,  &me.ev,       &me.te,       &me.vv_eid,
+                         &me.et_off,   &me.et_vals,  &me.face_off, &me.vf_vals,  &me.fv,       &me.tf,
+                         &me.fe,       &me.ft,       &me.ef_off,   &me.ef_vals,  &me.tt_off,   &me.tt_vals,
+                         &me.ff_off,   &me.ff_vals,  &me.ee_off,   &me.ee_vals,  &me.bitmap,   &me.rank_tmp,
+                         &me.tmp_a,    &me.tmp_b,    &me.tmp_c,    &me.bk_h1,    &me.bk_h2,    &me.bk_rowsrc,
+                         &me.bk_fixd,  &me.bk_rowdesc};
+    for (auto *a : all) {
+        a->stream = s;
+        if (release) a->release();
+    }
+    if (!release) return;
+    for (auto &e : me.marks.ev)
         if (e) {
             (void)hipEventDestroy(e);
             e = nullptr;

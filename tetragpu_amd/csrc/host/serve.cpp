@@ -36,6 +36,35 @@ int64_t footprint(int64_t T, int64_t V, uint32_t followups) {
 
 struct Group;
 
+// Result storage in pinned host memory from the library's reusing cache, so
+// large relations come back at full PCIe rate without page-locking per call.
+template <typename T>
+struct Pinned {
+    T *p = nullptr;
+    size_t n = 0, cap = 0;
+    Pinned() = default;
+    Pinned(const Pinned &) = delete;
+    Pinned &operator=(const Pinned &) = delete;
+    ~Pinned() { tg_host_free(p); }
+    bool resize(size_t k) {
+        n = k;
+        if (k <= cap) return true;
+        tg_host_free(p);
+        void *q = nullptr;
+        if (tg_host_alloc(std::max<size_t>(k, 1) * sizeof(T), &q) != TG_OK) {
+            p = nullptr;
+            n = cap = 0;
+            return false;
+        }
+        p = (T *)q;
+        cap = std::max<size_t>(k, 1);
+        return true;
+    }
+    T *data() { return p; }
+    const T *data() const { return p; }
+    T back() const { return p[n - 1]; }
+};
+
 struct Request {
     uint64_t id = 0;
     const int32_t *cells = nullptr;
@@ -49,9 +78,9 @@ struct Request {
     std::shared_ptr<Group> group;
     int32_t index = 0;       // position in the group
     int32_t group_size = 1;  // requests served by the same launch
-    std::vector<uint32_t> classes;
-    std::vector<int64_t> vt_off, tt_off;
-    std::vector<int32_t> vt_val, tt_val;
+    Pinned<uint32_t> classes;
+    Pinned<int64_t> vt_off, tt_off;
+    Pinned<int32_t> vt_val, tt_val;
     bool has_vt = false, has_tt = false;
 };
 
@@ -71,23 +100,21 @@ struct Group {
     }
 };
 
-int copy_csr(tg_mesh *m, int rel, std::vector<int64_t> &off, std::vector<int32_t> &val) {
+int copy_csr(tg_mesh *m, int rel, Pinned<int64_t> &off, Pinned<int32_t> &val) {
     int64_t rows = 0, nnz = 0;
     int32_t width = 0, fixed = 0;
     int rc = tg_relation_shape(m, rel, &rows, &nnz, &width, &fixed);
     if (rc) return rc;
-    off.resize((size_t)rows + 1);
-    val.resize((size_t)std::max<int64_t>(nnz * width, 1));
+    if (!off.resize((size_t)rows + 1) || !val.resize((size_t)(nnz * width))) return TG_ERR_ALLOC;
     return tg_relation_copy(m, rel, off.data(), val.data());
 }
 
-int copy_batch_csr(tg_batch *b, int32_t mesh, int rel, std::vector<int64_t> &off,
-                   std::vector<int32_t> &val, int64_t rows) {
+int copy_batch_csr(tg_batch *b, int32_t mesh, int rel, Pinned<int64_t> &off,
+                   Pinned<int32_t> &val, int64_t rows) {
     int64_t nnz = 0;
     int rc = tg_batch_relation_copy(b, mesh, rel, nullptr, nullptr, &nnz);
     if (rc) return rc;
-    off.resize((size_t)rows + 1);
-    val.resize((size_t)std::max<int64_t>(nnz, 1));
+    if (!off.resize((size_t)rows + 1) || !val.resize((size_t)nnz)) return TG_ERR_ALLOC;
     return tg_batch_relation_copy(b, mesh, rel, off.data(), val.data(), &nnz);
 }
 
@@ -181,7 +208,7 @@ int run_group(std::vector<Request *> &reqs, Group &g) {
         Request &r = *reqs[0];
         rc = tg_mesh_create(&g.mesh, r.cells, r.T, r.V, r.scalars, 0, 0, g.stream);
         if (!rc) rc = tg_vv_critpoints(g.mesh);
-        r.classes.resize(3 * (size_t)std::max<int64_t>(r.V, 1));
+        if (!rc && !r.classes.resize(3 * (size_t)std::max<int64_t>(r.V, 1))) rc = TG_ERR_ALLOC;
         if (!rc) rc = tg_classes_copy(g.mesh, r.classes.data());
     } else {
         const int32_t n = (int32_t)reqs.size();
@@ -203,8 +230,13 @@ int run_group(std::vector<Request *> &reqs, Group &g) {
         if (!rc) rc = tg_batch_classes_copy(g.batch, all.data());
         for (int32_t i = 0; i < n && !rc; i++) {
             reqs[i]->index = i;
-            reqs[i]->classes.assign(all.begin() + 3 * poff[i], all.begin() + 3 * poff[i + 1]);
-            if (reqs[i]->classes.empty()) reqs[i]->classes.resize(3);
+            const size_t k = 3 * (size_t)(poff[i + 1] - poff[i]);
+            if (!reqs[i]->classes.resize(std::max<size_t>(k, 3))) {
+                rc = TG_ERR_ALLOC;
+                break;
+            }
+            std::copy(all.begin() + 3 * poff[i], all.begin() + 3 * poff[i] + k,
+                      reqs[i]->classes.data());
         }
     }
     if (!rc && reqs[0]->followups) {
