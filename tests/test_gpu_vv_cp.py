@@ -11,7 +11,7 @@ from oracle import binding as O
 pytestmark = pytest.mark.gpu
 
 
-VARIANTS = [1, 2]  # vertex-lane form, bucketed form
+VARIANTS = [1, 2, 3]  # vertex-lane form, bucketed form, bucketed owner form
 
 
 def run_device(t, cells, n_points, scalars, cfg=None, twice=False, variant=0):
@@ -65,8 +65,8 @@ def test_irregular_hubs_over_64_neighbours(gpu):
     m = gpu.kuhn(12, seed=11, split_fraction=0.15, n_hubs=6, hub_splits=60, scalar="sines")
     VV = O.vv(m.cells, m.n_points)
     assert np.diff(VV.off).max() > 64
-    check(gpu, m.cells, m.n_points, m.scalars, variant=1)
-    check(gpu, m.cells, m.n_points, m.scalars, variant=2)
+    for variant in VARIANTS:
+        check(gpu, m.cells, m.n_points, m.scalars, variant=variant)
     check(gpu, m.cells, m.n_points, m.scalars, cfg=(64, 128))
 
 
@@ -157,11 +157,20 @@ def test_hub_vertices_bucket_overflow(gpu, variant):
     check(gpu, m.cells, m.n_points, m.scalars, variant=variant)
 
 
-def test_bucket_boundaries_partial_last_bucket(gpu):
+@pytest.mark.parametrize("variant", [2, 3])
+def test_bucket_boundaries_partial_last_bucket(gpu, variant):
     # V not a multiple of the bucket size, unreferenced trailing vertices
     m = gpu.kuhn(7, seed=23)
     check(gpu, m.cells, m.n_points + 77, np.concatenate([m.scalars, np.zeros(77, np.float32)]),
-          variant=2)
+          variant=variant)
+
+
+@pytest.mark.parametrize("variant", [2, 3])
+def test_dense_bucket_over_capacity(gpu, variant):
+    # one 128-vertex bucket receives ~25k corner records (more than the owner
+    # form's per-bucket index list holds): every vertex takes the serial path
+    cells, sc = soup(31, 200, 10000)
+    check(gpu, cells, 200, sc, variant=variant)
 
 
 def soup(seed, V, T):

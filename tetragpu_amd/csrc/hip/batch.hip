@@ -259,7 +259,7 @@ int tg_batch_set_config(tg_batch *b, int32_t variant, int32_t lane_cap, int32_t 
     if (!b) return fail(TG_ERR_ARG, "NULL batch");
     for (Wave &w : b->waves) {
         int rc = tg_mesh_set_config(w.mesh, variant, lane_cap, lane_block);
-        if (!rc && variant == 2) rc = tg_mesh_set_bucket(w.mesh, bv_log, tets_per_chunk);
+        if (!rc && variant >= 2) rc = tg_mesh_set_bucket(w.mesh, bv_log, tets_per_chunk);
         if (rc) return rc;
         if (w.exec) (void)hipGraphExecDestroy(w.exec);
         if (w.graph) (void)hipGraphDestroy(w.graph);

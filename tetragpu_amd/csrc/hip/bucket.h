@@ -14,6 +14,7 @@ struct BucketPlan {
     int32_t nchunks = 1;  // tet chunks of the first routing level
     int64_t tpc = 8192;   // tets per chunk
     int32_t slices = 1;   // slices per coarse bin in the second level
+    bool owner = false;   // bucket kernel: owner form (records grouped by vertex)
 };
 
 // Record arrays are 3 x 4T uint32 (structure of arrays).  rec1 aliases the

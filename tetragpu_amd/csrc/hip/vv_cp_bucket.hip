@@ -640,6 +640,234 @@ __global__ __launch_bounds__(2 << BV_LOG) void k_bkt_vv_cp(
     }
 }
 
+// Components of `mask` as label_components, over a lane-private adjacency
+// table stored slot-major (slot f of lane at adj[f * stride]).
+__device__ __forceinline__ uint32_t label_components_strided(const uint32_t *adj, int stride,
+                                                             uint32_t mask, uint32_t (&pl)[5]) {
+    uint32_t cnt = 0, rem = mask;
+    while (rem) {
+        uint32_t comp = rem & (0u - rem);
+        uint32_t front = comp;
+        while (front) {
+            const int f = __ffs(front) - 1;
+            front &= front - 1u;
+            const uint32_t nb = adj[f * stride] & mask & ~comp;
+            comp |= nb;
+            front |= nb;
+        }
+#pragma unroll
+        for (int k = 0; k < 5; k++)
+            if ((cnt >> k) & 1u) pl[k] |= comp;
+        rem &= ~comp;
+        cnt++;
+    }
+    return cnt;
+}
+
+// B, owner form: the bucket's records are first grouped by vertex with an
+// LDS counting sort (a list of record indices), then each lane owns one
+// vertex and builds its neighbour table and link adjacency alone, with plain
+// LDS reads and writes instead of contended compare-and-swap.  Tables are
+// slot-major (slot s of lane l at s * BV + l), so a wave's table accesses
+// fall on distinct banks.  Rows, descriptors and the rest are as in
+// k_bkt_vv_cp.  A bucket with more than OWN_CAP records sends all its
+// vertices to the serial kernel.
+constexpr int OWN_CAP = 8192;
+
+__device__ __forceinline__ uint32_t own_hash(int32_t u) { return ((uint32_t)u * 2654435769u) >> 27; }
+
+template <int BV_LOG, typename S>
+__global__ __launch_bounds__(2 << BV_LOG) void k_bkt_own(
+    const Rec rec, const int32_t *__restrict__ h2s, int32_t nslice,
+    const S *__restrict__ scal, int32_t V, int classify, int32_t *__restrict__ scratch,
+    int32_t *__restrict__ deg, int32_t *__restrict__ rowsrc, uint32_t *__restrict__ classes,
+    int32_t *__restrict__ heavy, int32_t *__restrict__ fixd, uint8_t *__restrict__ rowdesc,
+    int32_t *__restrict__ counters) {
+    constexpr int BV = 1 << BV_LOG;
+    constexpr int NT = 2 * BV;
+    __shared__ int32_t key[SLOTS * BV];   // slot-major; reused as row staging
+    __shared__ uint32_t adj[SLOTS * BV];
+    __shared__ uint16_t idx[OWN_CAP];
+    __shared__ int32_t cnt[BV + 1], cur[BV];
+    __shared__ int32_t wsum[2][NT / 64];
+
+    const int tid = threadIdx.x;
+    const int32_t b = blockIdx.x;
+    const int32_t v0 = b << BV_LOG;
+    const int nv = min(BV, V - v0);
+    const int32_t cbeg = h2s[(int64_t)b * nslice], cend = h2s[(int64_t)(b + 1) * nslice];
+    const int32_t nrec = cend - cbeg;
+    const bool fits = nrec <= OWN_CAP;
+
+    for (int i = tid; i < SLOTS * BV; i += NT) {
+        key[i] = -1;
+        adj[i] = 0u;
+    }
+    if (tid < BV) cnt[tid] = 0;
+    __syncthreads();
+    // counting sort of the bucket's records by vertex (indices into the run);
+    // the counts are needed even when the indices do not fit
+    for (int32_t i = tid; i < nrec; i += NT) atomicAdd(&cnt[rec.w0[cbeg + i] & (BV - 1)], 1);
+    __syncthreads();
+    if (tid < 64) {  // exclusive scan of cnt[0..BV) by the first wave
+        int32_t carry = 0;
+        for (int i0 = 0; i0 < BV; i0 += 64) {
+            const int32_t x = cnt[i0 + tid];
+            const int32_t inc = tgd::wave_inclusive_sum(x);
+            cur[i0 + tid] = carry + inc - x;
+            carry += __shfl(inc, 63, 64);
+        }
+        if (tid == 0) cnt[BV] = carry;
+    }
+    __syncthreads();
+    if (tid < BV) cnt[tid] = cur[tid];  // cnt = row starts, cnt[BV] = total
+    __syncthreads();
+    if (fits) {
+        for (int32_t i = tid; i < nrec; i += NT) {
+            const int p = atomicAdd(&cur[rec.w0[cbeg + i] & (BV - 1)], 1);
+            idx[p] = (uint16_t)i;
+        }
+    }
+    __syncthreads();
+
+    // phase 1 (owner): lane vl inserts the neighbours of vertex v0 + vl
+    const int vl = tid;
+    const bool act = vl < nv;
+    bool over = !fits;
+    uint32_t om = 0;
+    int d = 0;
+    int32_t nr = 0;
+    if (act) nr = cnt[vl + 1] - cnt[vl];
+    if (act && fits) {
+        const int p0 = cnt[vl], p1 = cnt[vl + 1];
+        auto insert = [&](int32_t u) -> int {
+            int s = (int)own_hash(u);
+#pragma unroll 1
+            for (int step = 0; step < SLOTS; step++, s = (s + 1) & (SLOTS - 1)) {
+                const int32_t k = key[s * BV + vl];
+                if (k == u) return s;
+                if (k < 0) {
+                    key[s * BV + vl] = u;
+                    om |= 1u << s;
+                    d++;
+                    return s;
+                }
+            }
+            return -1;
+        };
+        uint32_t a0 = 0, a1 = 0, a2 = 0;
+        if (p0 < p1) {
+            const int32_t i = cbeg + idx[p0];
+            a0 = rec.w0[i];
+            a1 = rec.w1[i];
+            a2 = rec.w2[i];
+        }
+        for (int p = p0; p < p1 && !over; p++) {
+            uint32_t rv, x, y, z;
+            rec_unpack(a0, a1, a2, rv, x, y, z);
+            if (p + 1 < p1) {  // next record in flight
+                const int32_t i = cbeg + idx[p + 1];
+                a0 = rec.w0[i];
+                a1 = rec.w1[i];
+                a2 = rec.w2[i];
+            }
+            const int s0 = insert((int32_t)x), s1 = insert((int32_t)y), s2 = insert((int32_t)z);
+            if ((s0 | s1 | s2) < 0) {
+                over = true;
+                break;
+            }
+            adj[s0 * BV + vl] |= (1u << s1) | (1u << s2);
+            adj[s1 * BV + vl] |= (1u << s0) | (1u << s2);
+            adj[s2 * BV + vl] |= (1u << s0) | (1u << s1);
+        }
+    }
+    const bool hv = act && over;
+    const bool ok = act && !over;
+
+    // phase 2-3 (owner): lower mask from the scalars, components, row keys
+    int32_t r[32];
+    uint32_t nl = 0, nu = 0;
+    if (ok) {
+        uint32_t lm = 0, pl[5] = {0, 0, 0, 0, 0};
+        if (classify) {
+            const int32_t v = v0 + vl;
+            const S sv = scal[v];
+            uint32_t rem = om;
+            while (rem) {
+                const int s = __ffs(rem) - 1;
+                rem &= rem - 1u;
+                const int32_t u = key[s * BV + vl];
+                if (is_lower(scal[u], sv, u, v)) lm |= 1u << s;
+            }
+            nl = label_components_strided(adj + vl, BV, lm, pl);
+            nu = label_components_strided(adj + vl, BV, om & ~lm, pl);
+        }
+#pragma unroll
+        for (int i = 0; i < 32; i++) {
+            const int32_t k = key[i * BV + vl];
+            const uint32_t desc = (((lm >> i) & 1u) << 5) | ((pl[0] >> i) & 1u) |
+                                  (((pl[1] >> i) & 1u) << 1) | (((pl[2] >> i) & 1u) << 2) |
+                                  (((pl[3] >> i) & 1u) << 3) | (((pl[4] >> i) & 1u) << 4);
+            r[i] = k < 0 ? INT32_MAX : (int32_t)(((uint32_t)k << 6) | desc);
+        }
+    }
+    // a heavy vertex reserves room for 3 neighbours per record (in a bucket
+    // whose index list did not fit, every vertex is heavy)
+    const int32_t hres = hv ? 3 * nr : 0;
+    __syncthreads();  // every lane has read its keys; key[] becomes row staging
+
+    const int32_t x = ok ? d : 0, y = hres;
+    const int32_t xi = tgd::wave_inclusive_sum(x), yi = tgd::wave_inclusive_sum(y);
+    const int lane = __lane_id(), wid = tid / 64;
+    if (lane == 63) {
+        wsum[0][wid] = xi;
+        wsum[1][wid] = yi;
+    }
+    __syncthreads();
+    int32_t xb = 0, yb = 0, xt = 0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; w++) {
+        if (w < wid) {
+            xb += wsum[0][w];
+            yb += wsum[1][w];
+        }
+        xt += wsum[0][w];
+    }
+    // rows and heavy reservations are laid out at 3 * cbeg; they fit in the
+    // bucket's 3 x nrec words of scratch (a row has at most 3 entries per
+    // record, a heavy reservation exactly that)
+    const int32_t loff = hv ? xt + yb + yi - y : xb + xi - x;
+    if (ok) {
+        sort32(r);
+#pragma unroll
+        for (int i = 0; i < 32; i++)
+            if (i < d) key[loff + i] = r[i];
+        const int32_t v = v0 + vl;
+        deg[v] = d;
+        if (classify) {
+            classes[3 * (int64_t)v] = nl;
+            classes[3 * (int64_t)v + 1] = nu;
+            classes[3 * (int64_t)v + 2] = verdict(nl, nu);
+        }
+    }
+    if (act) rowsrc[v0 + vl] = 3 * cbeg + loff;
+    wave_append(heavy, counters + C_HEAVY, v0 + vl, hv);
+    wave_append(fixd, counters + tg::C_FIXD, v0 + vl, ok && classify && (nl >= 2 || nu >= 2));
+    int dm = ok ? d : 0;
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) dm = max(dm, __shfl_xor(dm, s, 64));
+    if (lane == 0 && dm > 0) atomicMax(counters + C_MAXDEG, dm);
+    __syncthreads();
+
+    int32_t *dst = scratch + 3 * (int64_t)cbeg;
+    uint8_t *ddst = rowdesc + 3 * (int64_t)cbeg;
+    for (int i = tid; i < xt; i += NT) {
+        const int32_t pk = key[i];
+        dst[i] = pk >> 6;
+        if (classify) ddst[i] = (uint8_t)(pk & 63);
+    }
+}
+
 // Serial fallback for vertices with more than 32 neighbours: lane 0 scans
 // its bucket's corner records, keeping a 4096-entry neighbour list in LDS.
 template <int BV_LOG, typename S>
@@ -779,14 +1007,18 @@ int launch_bucketed_typed(tg::BucketBuffers &k, bool classify, hipStream_t s) {
                        p.fs, p.nb, k.h2, r2);
     TG_LAUNCH_CHECK("k_l2_scatter");
     if (mk) mk->mark("l2_scatter", s);
-    if (p.bv_log == 7)
-        hipLaunchKernelGGL((k_bkt_vv_cp<7, S>), dim3(p.nb), dim3(256), 0, s, r2, k.h2, p.slices,
-                           scal, V, cls, b.scratch, b.deg, k.rowsrc, b.classes, b.heavy, b.fixd,
-                           b.rowdesc, b.counters);
-    else
-        hipLaunchKernelGGL((k_bkt_vv_cp<8, S>), dim3(p.nb), dim3(512), 0, s, r2, k.h2, p.slices,
-                           scal, V, cls, b.scratch, b.deg, k.rowsrc, b.classes, b.heavy, b.fixd,
-                           b.rowdesc, b.counters);
+#define TG_BKT(KERNEL, BVL)                                                                  \
+    hipLaunchKernelGGL((KERNEL<BVL, S>), dim3(p.nb), dim3(2 << BVL), 0, s, r2, k.h2, p.slices, \
+                       scal, V, cls, b.scratch, b.deg, k.rowsrc, b.classes, b.heavy, b.fixd,   \
+                       b.rowdesc, b.counters)
+    if (p.owner) {
+        if (p.bv_log == 7) TG_BKT(k_bkt_own, 7);
+        else TG_BKT(k_bkt_own, 8);
+    } else {
+        if (p.bv_log == 7) TG_BKT(k_bkt_vv_cp, 7);
+        else TG_BKT(k_bkt_vv_cp, 8);
+    }
+#undef TG_BKT
     TG_LAUNCH_CHECK("k_bkt_vv_cp");
     if (mk) mk->mark("bkt_vv_cp", s);
     if (p.bv_log == 7)

@@ -21,14 +21,15 @@ std::vector<tg_tune_entry> candidates() {
             if (cap == 64 && blk == 256) continue;
             c.push_back({0, 1, cap, blk, 7, 8192});
         }
-    for (int bv : {7, 8})
-        for (int64_t tpc : {4096, 8192, 16384}) c.push_back({0, 2, 32, 128, bv, tpc});
+    for (int variant : {2, 3})
+        for (int bv : {7, 8})
+            for (int64_t tpc : {4096, 8192, 16384}) c.push_back({0, variant, 32, 128, bv, tpc});
     return c;
 }
 
 double time_config(tg_mesh *m, const tg_tune_entry &c, int reps) {
     if (tg_mesh_set_config(m, c.variant, c.lane_cap, c.lane_block) ||
-        (c.variant == 2 && tg_mesh_set_bucket(m, c.bv_log, c.tets_per_chunk)))
+        (c.variant >= 2 && tg_mesh_set_bucket(m, c.bv_log, c.tets_per_chunk)))
         return -1;
     if (tg_vv_critpoints(m) || tg_sync(m)) return -1;  // warm-up (allocations)
     std::vector<double> ms;
