@@ -102,6 +102,15 @@ int tg_mesh_set_config(tg_mesh *m, int32_t variant, int32_t lane_cap,
  * it.  Chosen by the autotuner per size class. */
 int tg_mesh_set_bucket(tg_mesh *m, int32_t bv_log, int64_t tets_per_chunk);
 
+/* Routing of the bucketed form's corner records: 0 automatic (decided on
+ * the mesh's first run: one level when the corners of 1024 consecutive tets
+ * span fewer than 1024 buckets in at least 63 of 64 such tiles, else two
+ * levels), 1 one level (tiles -> buckets), 2 two levels (chunks -> coarse
+ * bins -> buckets).  Results do not depend on it.  tg_mesh_routing reports
+ * the routing in use (0 until the first bucketed run in automatic mode). */
+int tg_mesh_set_routing(tg_mesh *m, int32_t route);
+int tg_mesh_routing(tg_mesh *m, int32_t *route);
+
 /* Process-wide tuning table used by variant 0 (automatic): the first entry
  * (ascending max_cells) with n_cells <= max_cells gives the kernel form and
  * its shape.  Written by the autotuner (include/tetra/autotune.h); no

@@ -11,7 +11,11 @@ from oracle import binding as O
 pytestmark = pytest.mark.gpu
 
 
-VARIANTS = [1, 2, 3]  # vertex-lane form, bucketed form, bucketed owner form
+# (variant, routing): vertex-lane form; bucketed form and bucketed owner form,
+# each with one-level and two-level routing of the corner records
+VARIANTS = [(1, 0), (2, 1), (2, 2), (3, 1), (3, 2)]
+IDS = ["lane", "bucket-1level", "bucket-2level", "owner-1level", "owner-2level"]
+BUCKETED, BUCKETED_IDS = VARIANTS[1:], IDS[1:]
 
 
 def run_device(t, cells, n_points, scalars, cfg=None, twice=False, variant=0):
@@ -19,7 +23,10 @@ def run_device(t, cells, n_points, scalars, cfg=None, twice=False, variant=0):
         if cfg:
             m.set_config(*cfg, variant=1)
         elif variant:
-            m.set_config(variant=variant)
+            v, route = variant if isinstance(variant, tuple) else (variant, 0)
+            m.set_config(variant=v)
+            if route:
+                m.set_routing(route)
         m.vv_critpoints()
         if twice:
             m.vv_critpoints()
@@ -39,7 +46,7 @@ def check(t, cells, n_points, scalars, cfg=None, twice=False, variant=0):
     return cls
 
 
-@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("variant", VARIANTS, ids=IDS)
 def test_config1_kuhn24(gpu, variant):
     m = gpu.kuhn(24, seed=1234)
     cls = check(gpu, m.cells, m.n_points, m.scalars, variant=variant)
@@ -53,7 +60,7 @@ def test_lane_shapes_identical(gpu, cfg):
     check(gpu, m.cells, m.n_points, m.scalars, cfg=cfg)
 
 
-@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("variant", VARIANTS, ids=IDS)
 @pytest.mark.parametrize("order", ["path", "shuffled"])
 def test_unsorted_cells(gpu, order, variant):
     m = gpu.kuhn(10, seed=3, order=order)
@@ -70,13 +77,13 @@ def test_irregular_hubs_over_64_neighbours(gpu):
     check(gpu, m.cells, m.n_points, m.scalars, cfg=(64, 128))
 
 
-@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("variant", VARIANTS, ids=IDS)
 def test_all_ties_constant_field(gpu, variant):
     m = gpu.kuhn(8, seed=5, scalar="constant")
     check(gpu, m.cells, m.n_points, m.scalars, variant=variant)
 
 
-@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("variant", VARIANTS, ids=IDS)
 def test_linear_field_one_min_one_max(gpu, variant):
     m = gpu.kuhn(9, permute_vertices=False, permute_cells=True, jitter=False, scalar="linear")
     cls = check(gpu, m.cells, m.n_points, m.scalars, variant=variant)
@@ -84,13 +91,13 @@ def test_linear_field_one_min_one_max(gpu, variant):
     assert (cls[:, 2] == 3).sum() == m.n_points - 2
 
 
-@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("variant", VARIANTS, ids=IDS)
 def test_gaussian_field_f64_scalars(gpu, variant):
     m = gpu.kuhn(14, seed=21, scalar="gaussians")
     check(gpu, m.cells, m.n_points, m.scalars.astype(np.float64) * (1 + 1e-9), variant=variant)
 
 
-@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("variant", VARIANTS, ids=IDS)
 def test_nan_scalars(gpu, variant):
     m = gpu.kuhn(6, seed=2)
     s = m.scalars.copy()
@@ -98,7 +105,7 @@ def test_nan_scalars(gpu, variant):
     check(gpu, m.cells, m.n_points, s, variant=variant)
 
 
-@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("variant", VARIANTS, ids=IDS)
 def test_isolated_and_tiny(gpu, variant):
     # one tet plus three vertices no cell references: (0, 0, 4)
     cells = np.array([[0, 2, 4, 5]], np.int32)
@@ -106,7 +113,7 @@ def test_isolated_and_tiny(gpu, variant):
     assert cls[1].tolist() == [0, 0, 4] and cls[3].tolist() == [0, 0, 4]
 
 
-@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("variant", VARIANTS, ids=IDS)
 def test_empty_mesh(gpu, variant):
     off, cols, cls = run_device(gpu, np.zeros((0, 4), np.int32), 5, np.zeros(5, np.float32),
                                 variant=variant)
@@ -114,7 +121,7 @@ def test_empty_mesh(gpu, variant):
     assert cls.tolist() == [[0, 0, 4]] * 5
 
 
-@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("variant", VARIANTS, ids=IDS)
 def test_repeat_is_deterministic(gpu, variant):
     m = gpu.kuhn(16, seed=9)
     a = run_device(gpu, m.cells, m.n_points, m.scalars, variant=variant)
@@ -149,7 +156,7 @@ def test_labels6_match_oracle(gpu):
     assert np.array_equal(lab, O.label6(cls))
 
 
-@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("variant", VARIANTS, ids=IDS)
 def test_hub_vertices_bucket_overflow(gpu, variant):
     # many vertices above 32 neighbours: the bucketed form routes them to its
     # serial kernel; both forms must agree with the oracle
@@ -157,7 +164,7 @@ def test_hub_vertices_bucket_overflow(gpu, variant):
     check(gpu, m.cells, m.n_points, m.scalars, variant=variant)
 
 
-@pytest.mark.parametrize("variant", [2, 3])
+@pytest.mark.parametrize("variant", BUCKETED, ids=BUCKETED_IDS)
 def test_bucket_boundaries_partial_last_bucket(gpu, variant):
     # V not a multiple of the bucket size, unreferenced trailing vertices
     m = gpu.kuhn(7, seed=23)
@@ -165,7 +172,7 @@ def test_bucket_boundaries_partial_last_bucket(gpu, variant):
           variant=variant)
 
 
-@pytest.mark.parametrize("variant", [2, 3])
+@pytest.mark.parametrize("variant", BUCKETED, ids=BUCKETED_IDS)
 def test_dense_bucket_over_capacity(gpu, variant):
     # one 128-vertex bucket receives ~25k corner records (more than the owner
     # form's per-bucket index list holds): every vertex takes the serial path
@@ -183,7 +190,7 @@ def soup(seed, V, T):
     return cells, sc
 
 
-@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("variant", VARIANTS, ids=IDS)
 @pytest.mark.parametrize("seed,V,T", [(1, 30, 40), (2, 200, 300), (3, 60, 400),
                                       (4, 40, 1500), (5, 400, 700)])
 def test_random_soup(gpu, variant, seed, V, T):
@@ -200,3 +207,28 @@ def test_non_flag_link_merges_components(gpu):
         cls = check(gpu, cells, 9, sc, variant=variant)
         # one lower component after the merge: a maximum, not a saddle
         assert cls[0].tolist() == [1, 0, 2]
+
+
+def test_routing_auto_decision_and_parity(gpu):
+    # grid-ordered vertices and cells: every 1024-tet tile spans few buckets
+    # -> one level; with seeded vertex and cell permutations every tile spans
+    # all buckets -> two levels.  Classes and VV agree across forced routings
+    # and the vertex-lane form.
+    for permuted, want in ((False, 1), (True, 2)):
+        m = gpu.kuhn(56, seed=29, permute_vertices=permuted, permute_cells=permuted,
+                     scalar="gaussians")
+        assert m.n_points > 1024 * 128
+        outs = []
+        for form in ((1, 0), (2, 0), (2, 1), (2, 2)):
+            with gpu.Mesh(m.cells, m.n_points, m.scalars) as d:
+                d.set_config(variant=form[0])
+                d.set_routing(form[1])
+                d.vv_critpoints()
+                if form == (2, 0):
+                    assert d.routing() == want, permuted
+                off, vals = d.relation("VV")
+                outs.append((off, vals[:, 0].copy(), d.classes()))
+        print(f"routing permuted={permuted} V={m.n_points} T={len(m.cells)}", flush=True)
+        for o in outs[1:]:
+            for x, y in zip(outs[0], o):
+                assert np.array_equal(x, y)

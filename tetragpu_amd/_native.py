@@ -55,6 +55,8 @@ _HIP_SIGS = {
     "tg_relation_export": (C.c_int, [P, I32, I32, C.POINTER(P), I32, C.POINTER(P)]),
     "tg_device_upload": (C.c_int, [P, C.c_size_t, C.POINTER(P)]),
     "tg_mesh_set_bucket": (C.c_int, [P, I32, I64]),
+    "tg_mesh_set_routing": (C.c_int, [P, I32]),
+    "tg_mesh_routing": (C.c_int, [P, C.POINTER(I32)]),
     "tg_set_tuning": (C.c_int, [P, I32]),
     "tg_get_tuning": (C.c_int, [P, I32, C.POINTER(I32)]),
     "tg_batch_create": (C.c_int, [C.POINTER(P), I32, P, P, P, P, C.c_int, I64, P]),

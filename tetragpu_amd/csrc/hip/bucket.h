@@ -17,14 +17,24 @@ struct BucketPlan {
     bool owner = false;   // bucket kernel: owner form (records grouped by vertex)
 };
 
+// Routing of the corner records to buckets.
+enum Route : int {
+    ROUTE_AUTO = 0,  // decide on the first run from the tiles' bucket spans
+    ROUTE_ONE = 1,   // tiles of tets -> buckets through an LDS window
+    ROUTE_TWO = 2,   // chunks -> coarse bins -> buckets
+};
+
 // Record arrays are 3 x 4T uint32 (structure of arrays).  rec1 aliases the
 // padded row scratch and rec2 the CSR columns: both are dead while the
 // records live.
 struct BucketBuffers {
     VvCpBuffers base;
     BucketPlan plan;
-    int32_t *h1 = nullptr;      // [nc*nchunks + 1] level-1 counts -> offsets
+    int32_t *route = nullptr;   // the mesh's routing (Route); AUTO is replaced on first run
+    int32_t *h1 = nullptr;      // [max(nc*nchunks, nb) + 1] level-1 counts -> offsets
+                                //   (one level: per-bucket fill counters)
     int32_t *h2 = nullptr;      // [nb*slices + 1] level-2 counts -> offsets
+                                //   (one level: bucket starts, [nb + 1])
     uint32_t *rec1 = nullptr;   // level-1 output
     uint32_t *rec2 = nullptr;   // level-2 output (bucket order)
     int32_t *rowsrc = nullptr;  // [V] row start in scratch

@@ -258,6 +258,22 @@ int tg_mesh_set_bucket(tg_mesh *m, int32_t bv_log, int64_t tets_per_chunk) {
     return TG_OK;
 }
 
+int tg_mesh_set_routing(tg_mesh *m, int32_t route) {
+    if (!m) return fail(TG_ERR_ARG, "NULL mesh");
+    if (route < tg::ROUTE_AUTO || route > tg::ROUTE_TWO)
+        return fail(TG_ERR_ARG, "route must be 0 (auto), 1 (one level) or 2 (two levels)");
+    m->cfg.route = route;
+    m->hot_ready = false;  // re-plan (and re-decide in auto mode) on the next run
+    m->bk_ready = false;
+    return TG_OK;
+}
+
+int tg_mesh_routing(tg_mesh *m, int32_t *route) {
+    if (!m || !route) return fail(TG_ERR_ARG, "NULL argument");
+    *route = m->hot_ready ? m->bk_route : m->cfg.route;
+    return TG_OK;
+}
+
 int tg_set_tuning(const tg_tune_entry *entries, int32_t n) {
     if (n < 0 || (n > 0 && !entries)) return fail(TG_ERR_ARG, "bad tuning table");
     return tg::set_tuning(entries, n);
@@ -402,7 +418,7 @@ int tg_approx_max_vv(tg_mesh *m, int32_t *max_vv) {
     if (!m || !max_vv) return fail(TG_ERR_ARG, "NULL argument");
     int rc = tg::ensure_vt_corners(*m);
     if (rc) return rc;
-    int32_t *d = m->counters.as<int32_t>() + tg::N_COUNTERS - 1;
+    int32_t *d = m->counters.as<int32_t>() + tg::C_SCRATCH;
     if ((rc = tg::fill_i32(d, 0, 1, m->stream))) return rc;
     if (m->n_points > 0) {
         unsigned g = std::min<unsigned>(tg::grid_for(m->n_points, 256), 4096u);

@@ -1,7 +1,9 @@
 // Host launchers of the relation and critical-point pipelines.  Every
 // launcher only enqueues work on the given stream (no host synchronisation,
 // no allocation, no hipMemset), so a launch sequence can be captured into a
-// hipGraph and replayed.
+// hipGraph and replayed.  One exception: the first bucketed run of a mesh
+// waits once for the routing decision (one-level or two-level), which is
+// then kept with the mesh; a run under capture never waits.
 #pragma once
 
 #include "common.h"
@@ -18,7 +20,9 @@ enum Counter : int {
     C_FIXD = 5,    // bucketed vertices needing the link pass (row descriptors)
     C_FTERR = 6,   // a face held by more than two cells
     C_IDERR = 7,   // face ids: a relation read back inconsistent
-    N_COUNTERS = 8
+    C_WIDE = 8,    // one-level routing: tiles whose corners span too many buckets
+    C_SCRATCH = 9, // one-off reductions of the host API
+    N_COUNTERS = 10
 };
 enum ErrBit : int { ERRBIT_CAPACITY = 1 };
 
@@ -39,12 +43,13 @@ struct StageMarks {
 
 // Tunables of the VV + critical point pipeline (picked by the autotuner).
 struct VvCpConfig {
-    int variant = 0;       // 0 auto, 1 vertex-lane form, 2 bucketed form
+    int variant = 0;       // 0 auto, 1 vertex-lane form, 2 bucketed, 3 bucketed owner form
     int lane_cap = 32;     // neighbours held per lane (32 or 64)
     int lane_block = 128;  // lanes (vertices) per workgroup
     int corner_block = 256;
     int bv_log = 7;        // bucketed form: 2^bv_log vertices per bucket (7 or 8)
     int64_t tpc = 8192;    // bucketed form: tets per first-level chunk
+    int route = 0;         // bucketed form: 0 auto, 1 one-level, 2 two-level routing
     bool tuned = false;    // variant 0 resolved from the tuning table
 };
 

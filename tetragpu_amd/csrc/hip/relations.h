@@ -44,6 +44,7 @@ struct tg_mesh {
         heavy, fix, fixbig, counters, scan_tmp, labels;
     tg::DevArr bk_h1, bk_h2, bk_rowsrc, bk_fixd, bk_rowdesc;
     tg::BucketPlan bplan;
+    int32_t bk_route = 0;  // tg::Route of the bucketed pipeline (AUTO until the first run)
     bool vt_ready = false, bk_ready = false;
     bool hot_ready = false;         // buffers sized for the hot path
     bool vt_corners_built = false;  // vt_off / vtc valid

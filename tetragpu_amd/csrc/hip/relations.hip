@@ -109,6 +109,7 @@ int prepare_hot_path(tg_mesh &m) {
         return rc;
     resolve_tuning(m);
     m.bplan = plan_buckets(m.n_cells, m.n_points, m.cfg.bv_log, m.cfg.tpc);
+    m.bk_route = m.cfg.route;
     m.hot_ready = true;
     return OK;
 }
@@ -175,7 +176,8 @@ int prepare_buckets(tg_mesh &m) {
     if (rc) return rc;
     const BucketPlan &p = m.bplan;
     const size_t I = sizeof(int32_t);
-    const size_t n1 = (size_t)p.nc * (size_t)p.nchunks + 1, n2 = (size_t)p.nb * p.slices + 1;
+    const size_t n1 = std::max((size_t)p.nc * (size_t)p.nchunks, (size_t)p.nb) + 1,
+                 n2 = (size_t)p.nb * p.slices + 1;
     if ((rc = m.bk_h1.ensure(I * n1)) || (rc = m.bk_h2.ensure(I * n2)) ||
         (rc = m.bk_rowsrc.ensure(I * (size_t)m.n_points)) ||
         (rc = m.bk_fixd.ensure(I * (size_t)m.n_points)) ||
@@ -196,6 +198,7 @@ static BucketBuffers bucket_buffers(tg_mesh &m) {
     BucketBuffers k;
     k.base = hot_buffers(m);
     k.plan = m.bplan;
+    k.route = &m.bk_route;
     k.h1 = m.bk_h1.as<int32_t>();
     k.h2 = m.bk_h2.as<int32_t>();
     k.rec1 = m.scratch.as<uint32_t>();

@@ -615,12 +615,12 @@ __global__ __launch_bounds__(64) void k_cp_fix_desc(
                 for (int k = 0; k < 16; k++) c[k] = j0 + k < re ? cols[j0 + k] : INT32_MAX;
 #pragma unroll
                 for (int k = 0; k < 16; k++) {
-                    int lo = 0, hi = d;
-                    while (lo < hi) {
-                        const int mid = (lo + hi) >> 1;
-                        if (nbs[half][mid] < c[k]) lo = mid + 1;
-                        else hi = mid;
-                    }
+                    // lower bound in the 32 sorted entries (INT32_MAX past d):
+                    // fixed steps, so the 16 searches overlap their LDS reads
+                    int lo = 0;
+#pragma unroll
+                    for (int step = 16; step >= 1; step >>= 1)
+                        if (nbs[half][lo + step - 1] < c[k]) lo += step;
                     if (lo < d && nbs[half][lo] == c[k]) {
                         const int dk = dsc[half][lo];
                         const int ck = dk & 31;

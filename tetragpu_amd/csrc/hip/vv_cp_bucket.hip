@@ -308,10 +308,182 @@ __global__ __launch_bounds__(L2_THREADS) void k_l2_scatter(const Rec in,
     }
 }
 
+// ---- one-level routing: tiles of tets -> buckets through a window -----------
+// When consecutive tets reference nearby vertex ids (meshes written in a
+// spatial order, structured grids), the corners of one tile of S1_TILE tets
+// fall into fewer than S1_WIN consecutive buckets.  Such a tile is counting-
+// sorted in LDS by (bucket - first bucket of the tile) and every non-empty
+// run takes its place in the bucket's range with one global atomic per
+// (tile, bucket); the order of records inside a bucket does not matter to
+// the bucket kernels.  A wider tile places each corner with its own global
+// atomic.  k_s1_count counts the wide tiles; a mesh where they are more than
+// 1/64 of all tiles is routed in two levels instead.
+constexpr int S1_THREADS = 1024;     // one tet per thread
+constexpr int S1_TILE = S1_THREADS;  // tets per tile
+constexpr int S1_WIN = S1_THREADS;   // buckets per window (one per thread)
+constexpr int S1_NSUB = 4;           // count sub-histograms
+constexpr int S1_NW = S1_THREADS / 64;
+
+// Smallest and largest bucket of the tile (threads without a tet pass
+// INT32_MAX / -1).  Contains a __syncthreads.
+__device__ __forceinline__ void tile_window(int32_t lo, int32_t hi, int32_t (&red)[2][S1_NW],
+                                            int32_t &bmin, int32_t &bmax) {
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) {
+        lo = min(lo, __shfl_xor(lo, s, 64));
+        hi = max(hi, __shfl_xor(hi, s, 64));
+    }
+    if (__lane_id() == 0) {
+        red[0][threadIdx.x / 64] = lo;
+        red[1][threadIdx.x / 64] = hi;
+    }
+    __syncthreads();
+    bmin = red[0][0];
+    bmax = red[1][0];
+#pragma unroll
+    for (int i = 1; i < S1_NW; i++) {
+        bmin = min(bmin, red[0][i]);
+        bmax = max(bmax, red[1][i]);
+    }
+}
+
+__global__ __launch_bounds__(S1_THREADS) void k_s1_count(const int4 *__restrict__ tv4, int64_t T,
+                                                          int64_t tpc, int bv_log,
+                                                          int32_t *__restrict__ btot,
+                                                          int32_t *__restrict__ counters) {
+    __shared__ int32_t h[S1_NSUB][S1_WIN];
+    __shared__ int32_t red[2][S1_NW];
+    const int tid = threadIdx.x;
+    int32_t *hs = h[(tid / 64) % S1_NSUB];
+    const int64_t t0 = (int64_t)blockIdx.x * tpc, t1 = min(T, t0 + tpc);
+    int4 qn = make_int4(0, 0, 0, 0);
+    if (t0 + tid < t1) qn = tv4[t0 + tid];
+    for (int64_t tb = t0; tb < t1; tb += S1_TILE) {
+        const bool on = tb + tid < t1;
+        const int4 q = qn;
+        if (tb + S1_TILE + tid < t1) qn = tv4[tb + S1_TILE + tid];
+        const int32_t b0 = q.x >> bv_log, b1 = q.y >> bv_log, b2 = q.z >> bv_log,
+                      b3 = q.w >> bv_log;
+#pragma unroll
+        for (int k = 0; k < S1_NSUB; k++) h[k][tid] = 0;
+        int32_t bmin, bmax;
+        tile_window(on ? min(min(b0, b1), min(b2, b3)) : INT32_MAX,
+                    on ? max(max(b0, b1), max(b2, b3)) : -1, red, bmin, bmax);
+        if (bmax - bmin < S1_WIN) {
+            if (on) {
+                atomicAdd(&hs[b0 - bmin], 1);
+                atomicAdd(&hs[b1 - bmin], 1);
+                atomicAdd(&hs[b2 - bmin], 1);
+                atomicAdd(&hs[b3 - bmin], 1);
+            }
+            __syncthreads();
+            int32_t c = 0;
+#pragma unroll
+            for (int k = 0; k < S1_NSUB; k++) c += h[k][tid];
+            if (c) atomicAdd(&btot[bmin + tid], c);
+        } else {
+            if (on) {
+                atomicAdd(&btot[b0], 1);
+                atomicAdd(&btot[b1], 1);
+                atomicAdd(&btot[b2], 1);
+                atomicAdd(&btot[b3], 1);
+            }
+            if (tid == 0) atomicAdd(counters + tg::C_WIDE, 1);
+        }
+        __syncthreads();  // h and red are reused by the next tile
+    }
+}
+
+struct S1Stage {
+    uint32_t w0[4 * S1_TILE], w1[4 * S1_TILE], w2[4 * S1_TILE];
+    uint16_t bin[4 * S1_TILE];
+    int32_t cnt[S1_WIN], toff[S1_WIN], gb[S1_WIN];
+    int32_t red[2][S1_NW];
+    int32_t wsum[S1_NW];
+};
+
+// bstart: exclusive scan of the bucket counts; bused: zeroed fill counters.
+__global__ __launch_bounds__(S1_THREADS) void k_s1_scatter(const int4 *__restrict__ tv4,
+                                                            int64_t T, int64_t tpc, int bv_log,
+                                                            const int32_t *__restrict__ bstart,
+                                                            int32_t *__restrict__ bused,
+                                                            Rec out) {
+    __shared__ S1Stage st;
+    const int tid = threadIdx.x, lane = __lane_id(), wid = tid / 64;
+    const int64_t t0 = (int64_t)blockIdx.x * tpc, t1 = min(T, t0 + tpc);
+    int4 qn = make_int4(0, 0, 0, 0);
+    if (t0 + tid < t1) qn = tv4[t0 + tid];
+    for (int64_t tb = t0; tb < t1; tb += S1_TILE) {
+        const bool on = tb + tid < t1;
+        const int4 q = qn;
+        if (tb + S1_TILE + tid < t1) qn = tv4[tb + S1_TILE + tid];
+        const uint32_t x = (uint32_t)q.x, y = (uint32_t)q.y, z = (uint32_t)q.z, w = (uint32_t)q.w;
+        const int32_t b0 = q.x >> bv_log, b1 = q.y >> bv_log, b2 = q.z >> bv_log,
+                      b3 = q.w >> bv_log;
+        st.cnt[tid] = 0;
+        int32_t bmin, bmax;
+        tile_window(on ? min(min(b0, b1), min(b2, b3)) : INT32_MAX,
+                    on ? max(max(b0, b1), max(b2, b3)) : -1, st.red, bmin, bmax);
+        if (bmax - bmin < S1_WIN) {
+            int r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+            if (on) {
+                r0 = atomicAdd(&st.cnt[b0 - bmin], 1);
+                r1 = atomicAdd(&st.cnt[b1 - bmin], 1);
+                r2 = atomicAdd(&st.cnt[b2 - bmin], 1);
+                r3 = atomicAdd(&st.cnt[b3 - bmin], 1);
+            }
+            __syncthreads();
+            const int32_t c = st.cnt[tid];
+            // this tile's run of bucket bmin + tid: one global atomic
+            const int32_t base = c ? bstart[bmin + tid] + atomicAdd(&bused[bmin + tid], c) : 0;
+            const int32_t inc = tgd::wave_inclusive_sum(c);
+            if (lane == 63) st.wsum[wid] = inc;
+            __syncthreads();
+            int32_t pre = 0, total = 0;
+#pragma unroll
+            for (int i = 0; i < S1_NW; i++) {
+                if (i < wid) pre += st.wsum[i];
+                total += st.wsum[i];
+            }
+            st.toff[tid] = pre + inc - c;
+            st.gb[tid] = base;
+            __syncthreads();
+            if (on) {
+                auto put = [&](int j, int rank, uint32_t v, uint32_t a, uint32_t b, uint32_t cc) {
+                    const int slot = st.toff[j] + rank;
+                    st.w0[slot] = v | (a << 24);
+                    st.w1[slot] = (a >> 8) | (b << 16);
+                    st.w2[slot] = (b >> 16) | (cc << 8);
+                    st.bin[slot] = (uint16_t)j;
+                };
+                put(b0 - bmin, r0, x, y, z, w);
+                put(b1 - bmin, r1, y, x, z, w);
+                put(b2 - bmin, r2, z, x, y, w);
+                put(b3 - bmin, r3, w, x, y, z);
+            }
+            __syncthreads();
+            for (int slot = tid; slot < total; slot += S1_THREADS) {
+                const int j = st.bin[slot];
+                const int64_t pos = (int64_t)st.gb[j] + (slot - st.toff[j]);
+                out.w0[pos] = st.w0[slot];
+                out.w1[pos] = st.w1[slot];
+                out.w2[pos] = st.w2[slot];
+            }
+        } else if (on) {
+            rec_store(out, (int64_t)bstart[b0] + atomicAdd(&bused[b0], 1), x, y, z, w);
+            rec_store(out, (int64_t)bstart[b1] + atomicAdd(&bused[b1], 1), y, x, z, w);
+            rec_store(out, (int64_t)bstart[b2] + atomicAdd(&bused[b2], 1), z, x, y, w);
+            rec_store(out, (int64_t)bstart[b3] + atomicAdd(&bused[b3], 1), w, x, y, z);
+        }
+        __syncthreads();  // the stage is reused by the next tile
+    }
+}
+
 // ---- B: per-bucket neighbour sets, link components, sorted rows --------------
 // Probe order of key u in a 32-slot table: 8 groups of 4 slots (one 16-byte
 // LDS read per group), starting at the home group from a multiplicative
-// hash; inside a group a key claims the first empty slot.
+// hash; inside a group a key claims the first empty slot at or after its
+// hint slot (cyclic), so two new keys of one group seldom race for a slot.
 __device__ __forceinline__ uint32_t key_hash(int32_t u) {
     return ((uint32_t)u * 2654435769u) >> 27;  // 5 bits: group (3) | slot hint (2)
 }
@@ -327,6 +499,9 @@ __device__ __forceinline__ int first_empty_from(const int4 q, int p) {
 // Insert u into a vertex's table (lock-free: slots only go from empty (-1)
 // to a key by CAS, and every thread inserting u follows the same probe
 // order, so u lands in exactly one slot).  Returns the slot or -1 if full.
+// HINT: start each group at the key's hint slot (else at slot 0); every
+// insert into one table must use the same setting.
+template <bool HINT>
 __device__ __forceinline__ int hash_insert(int32_t *tab, int32_t u) {
     const uint32_t h = key_hash(u);
 #pragma unroll 1
@@ -340,7 +515,7 @@ __device__ __forceinline__ int hash_insert(int32_t *tab, int32_t u) {
             if (q.y == u) return 4 * g + 1;
             if (q.z == u) return 4 * g + 2;
             if (q.w == u) return 4 * g + 3;
-            const int e = first_empty_from(q, 0);
+            const int e = first_empty_from(q, HINT ? (int)(h & 3u) : 0);
             if (e < 0) break;
             const int32_t old = atomicCAS(&grp[e], -1, u);
             if (old == -1 || old == u) return 4 * g + e;
@@ -351,10 +526,11 @@ __device__ __forceinline__ int hash_insert(int32_t *tab, int32_t u) {
 }
 
 // Components of `mask` under the slot adjacency, numbered 0.. in order of
-// their lowest slot; the number of every slot's component is recorded in
-// five bit-planes (bit k of slot i's number is bit i of pl[k]).
-__device__ __forceinline__ uint32_t label_components(const uint32_t *adj, uint32_t mask,
-                                                     uint32_t (&pl)[5]) {
+// their lowest slot.  Every slot of `mask` is visited once, and its key is
+// rewritten in place as (key << 6) | tag | component number: the row's sort
+// key and its descriptor (lower bit 5, component bits 0-4) in one word.
+__device__ __forceinline__ uint32_t label_components(const uint32_t *adj, int32_t *key,
+                                                     uint32_t mask, uint32_t tag) {
     uint32_t cnt = 0, rem = mask;
     while (rem) {
         uint32_t comp = rem & (0u - rem);
@@ -363,12 +539,10 @@ __device__ __forceinline__ uint32_t label_components(const uint32_t *adj, uint32
             const int f = __ffs(front) - 1;
             front &= front - 1u;
             const uint32_t nb = adj[f] & mask & ~comp;
+            key[f] = (int32_t)(((uint32_t)key[f] << 6) | tag | cnt);
             comp |= nb;
             front |= nb;
         }
-#pragma unroll
-        for (int k = 0; k < 5; k++)
-            if ((cnt >> k) & 1u) pl[k] |= comp;
         rem &= ~comp;
         cnt++;
     }
@@ -376,7 +550,8 @@ __device__ __forceinline__ uint32_t label_components(const uint32_t *adj, uint32
 }
 
 // Bitonic sorting network over 32 registers (ascending).
-__device__ __forceinline__ void sort32(int32_t (&r)[32]) {
+template <typename K>
+__device__ __forceinline__ void sort32(K (&r)[32]) {
 #pragma unroll
     for (int k = 2; k <= 32; k <<= 1)
 #pragma unroll
@@ -385,7 +560,7 @@ __device__ __forceinline__ void sort32(int32_t (&r)[32]) {
             for (int i = 0; i < 32; i++) {
                 const int l = i ^ j;
                 if (l > i) {
-                    const int32_t a = r[i], b = r[l];
+                    const K a = r[i], b = r[l];
                     const bool up = (i & k) == 0;
                     r[i] = up ? min(a, b) : max(a, b);
                     r[l] = up ? max(a, b) : min(a, b);
@@ -404,8 +579,15 @@ __global__ __launch_bounds__(2 << BV_LOG) void k_bkt_vv_cp(
     constexpr int NT = 2 * BV;
     constexpr int BATCH = 4;
     constexpr int PER_T = BV * SLOTS / NT;  // slots per thread in phase 2 (16)
-    __shared__ __align__(16) int32_t key[BV * SLOTS];
-    __shared__ uint32_t adj[BV * SLOTS];
+    // table strides: a vertex's slots start 36 (keys, 16-byte groups) or 33
+    // (adjacency) words after the previous vertex's, so accesses of many
+    // vertices at one slot index spread over the LDS banks
+    constexpr int KS = SLOTS + 4, AS = SLOTS + 1;
+    // hint slots cut CAS conflicts but cost registers: at 512 threads they
+    // would halve the resident blocks (measured on MI355X, Kuhn 128^3)
+    constexpr bool HINT = BV_LOG == 7;
+    __shared__ __align__(16) int32_t key[BV * KS];
+    __shared__ uint32_t adj[BV * AS];
     __shared__ uint32_t occ[BV], lowm[BV];
     __shared__ int32_t ccount[BV];
     __shared__ int32_t ovf[BV];
@@ -418,10 +600,8 @@ __global__ __launch_bounds__(2 << BV_LOG) void k_bkt_vv_cp(
     const int nv = min(BV, V - v0);
     const int32_t cbeg = h2s[(int64_t)b * nslice], cend = h2s[(int64_t)(b + 1) * nslice];
 
-    for (int i = tid; i < BV * SLOTS; i += NT) {
-        key[i] = -1;
-        adj[i] = 0u;
-    }
+    for (int i = tid; i < BV * KS; i += NT) key[i] = -1;
+    for (int i = tid; i < BV * AS; i += NT) adj[i] = 0u;
     if (tid < BV) {
         occ[tid] = 0u;
         lowm[tid] = 0u;
@@ -466,7 +646,7 @@ __global__ __launch_bounds__(2 << BV_LOG) void k_bkt_vv_cp(
 #pragma unroll
         for (int j = 0; j < NK; j++) {
             const uint32_t g = key_hash(u[j]) >> 2;
-            grp[j] = *reinterpret_cast<const int4 *>(key + vls[j / 3] * SLOTS + 4 * g);
+            grp[j] = *reinterpret_cast<const int4 *>(key + vls[j / 3] * KS + 4 * g);
         }
         int sl[NK], cs[NK];
 #pragma unroll
@@ -476,7 +656,7 @@ __global__ __launch_bounds__(2 << BV_LOG) void k_bkt_vv_cp(
             const int g = (int)(h >> 2);
             const int4 q = grp[j];
             const int f = q.x == x ? 0 : q.y == x ? 1 : q.z == x ? 2 : q.w == x ? 3 : -1;
-            const int e = first_empty_from(q, 0);
+            const int e = first_empty_from(q, HINT ? (int)(h & 3u) : 0);
             const bool valid = w[j / 3][0] != 0xFFFFFFFFu;
             sl[j] = f >= 0 ? 4 * g + f : -1;
             cs[j] = (valid && f < 0 && e >= 0) ? 4 * g + e : -1;  // slot to claim
@@ -485,7 +665,7 @@ __global__ __launch_bounds__(2 << BV_LOG) void k_bkt_vv_cp(
         int32_t old[NK];
 #pragma unroll
         for (int j = 0; j < NK; j++)
-            old[j] = cs[j] >= 0 ? atomicCAS(key + vls[j / 3] * SLOTS + cs[j], -1, u[j]) : 0;
+            old[j] = cs[j] >= 0 ? atomicCAS(key + vls[j / 3] * KS + cs[j], -1, u[j]) : 0;
         uint32_t pending = 0;
 #pragma unroll
         for (int j = 0; j < NK; j++) {
@@ -504,7 +684,7 @@ __global__ __launch_bounds__(2 << BV_LOG) void k_bkt_vv_cp(
                         x = u[t];
                         vl = vls[t / 3];
                     }
-                const int r = hash_insert(key + vl * SLOTS, x);
+                const int r = hash_insert<HINT>(key + vl * KS, x);
 #pragma unroll
                 for (int t = 0; t < NK; t++)
                     if (t == j) sl[t] = r;
@@ -519,7 +699,7 @@ __global__ __launch_bounds__(2 << BV_LOG) void k_bkt_vv_cp(
             if ((a | b2 | c) < 0) {
                 ovf[vl] = 1;
             } else {
-                uint32_t *av = adj + vl * SLOTS;
+                uint32_t *av = adj + vl * AS;
                 atomicOr(&av[a], (1u << b2) | (1u << c));
                 atomicOr(&av[b2], (1u << a) | (1u << c));
                 atomicOr(&av[c], (1u << a) | (1u << b2));
@@ -529,7 +709,7 @@ __global__ __launch_bounds__(2 << BV_LOG) void k_bkt_vv_cp(
     __syncthreads();
     // occupancy masks from the final tables
     for (int vl = tid / SLOTS; vl < BV; vl += NT / SLOTS) {
-        const uint64_t bal = __ballot(key[vl * SLOTS + (tid & (SLOTS - 1))] >= 0);
+        const uint64_t bal = __ballot(key[vl * KS + (tid & (SLOTS - 1))] >= 0);
         if ((tid & (SLOTS - 1)) == 0) occ[vl] = (uint32_t)(bal >> (tid & 32));
     }
     __syncthreads();
@@ -542,7 +722,7 @@ __global__ __launch_bounds__(2 << BV_LOG) void k_bkt_vv_cp(
 #pragma unroll
         for (int k = 0; k < PER_T; k++) {
             const int idx = tid + k * NT;
-            u[k] = (idx / SLOTS < nv) ? key[idx] : -1;
+            u[k] = (idx / SLOTS < nv) ? key[(idx / SLOTS) * KS + idx % SLOTS] : -1;
         }
 #pragma unroll
         for (int k = 0; k < PER_T; k++) su[k] = u[k] >= 0 ? scal[u[k]] : S(0);
@@ -558,32 +738,31 @@ __global__ __launch_bounds__(2 << BV_LOG) void k_bkt_vv_cp(
     }
     __syncthreads();
 
-    // phase 3: per vertex (lanes < nv): components, row registers
+    // phase 3: per vertex (lanes < nv): components, row registers.  Sort
+    // keys are unsigned, so empty slots (-1) sort last.
     const int vl = tid;
     const bool act = vl < nv;
     const bool hv = act && ovf[vl] != 0;
     const bool ok = act && !hv;
-    int32_t r[32];
+    uint32_t r[32];
     uint32_t nl = 0, nu = 0;
     int d = 0;
     if (ok) {
         const uint32_t om = occ[vl];
         d = __popc(om);
-        uint32_t lm = 0, pl[5] = {0, 0, 0, 0, 0};
         if (classify) {
-            lm = lowm[vl] & om;
-            nl = label_components(adj + vl * SLOTS, lm, pl);
-            nu = label_components(adj + vl * SLOTS, om & ~lm, pl);
+            const uint32_t lm = lowm[vl] & om;
+            nl = label_components(adj + vl * AS, key + vl * KS, lm, 32u);
+            nu = label_components(adj + vl * AS, key + vl * KS, om & ~lm, 0u);
         }
-        // sort key: neighbour id, then (lower bit, component number) as the
-        // row descriptor the row-membership link pass reads
+        const uint4 *kv = reinterpret_cast<const uint4 *>(key + vl * KS);
 #pragma unroll
-        for (int i = 0; i < 32; i++) {
-            const int32_t k = key[vl * SLOTS + i];
-            const uint32_t desc = (((lm >> i) & 1u) << 5) | ((pl[0] >> i) & 1u) |
-                                  (((pl[1] >> i) & 1u) << 1) | (((pl[2] >> i) & 1u) << 2) |
-                                  (((pl[3] >> i) & 1u) << 3) | (((pl[4] >> i) & 1u) << 4);
-            r[i] = k < 0 ? INT32_MAX : (int32_t)(((uint32_t)k << 6) | desc);
+        for (int i = 0; i < SLOTS / 4; i++) {
+            const uint4 q = kv[i];
+            r[4 * i] = q.x;
+            r[4 * i + 1] = q.y;
+            r[4 * i + 2] = q.z;
+            r[4 * i + 3] = q.w;
         }
     }
     const int32_t hres = hv ? 3 * ccount[vl] : 0;
@@ -612,7 +791,7 @@ __global__ __launch_bounds__(2 << BV_LOG) void k_bkt_vv_cp(
         sort32(r);
 #pragma unroll
         for (int i = 0; i < 32; i++)
-            if (i < d) key[loff + i] = r[i];
+            if (i < d) key[loff + i] = (int32_t)r[i];
         const int32_t v = v0 + vl;
         deg[v] = d;
         if (classify) {
@@ -633,10 +812,14 @@ __global__ __launch_bounds__(2 << BV_LOG) void k_bkt_vv_cp(
     // phase 4: coalesced store of the bucket's rows and row descriptors
     int32_t *dst = scratch + 3 * (int64_t)cbeg;
     uint8_t *ddst = rowdesc + 3 * (int64_t)cbeg;
-    for (int i = tid; i < xt; i += NT) {
-        const int32_t pk = key[i];
-        dst[i] = pk >> 6;
-        if (classify) ddst[i] = (uint8_t)(pk & 63);
+    if (classify) {
+        for (int i = tid; i < xt; i += NT) {
+            const int32_t pk = key[i];
+            dst[i] = pk >> 6;
+            ddst[i] = (uint8_t)(pk & 63);
+        }
+    } else {
+        for (int i = tid; i < xt; i += NT) dst[i] = key[i];
     }
 }
 
@@ -985,30 +1168,73 @@ int launch_bucketed_typed(tg::BucketBuffers &k, bool classify, hipStream_t s) {
     const Rec r1{k.rec1, k.rec1 + n, k.rec1 + 2 * n};
     const Rec r2{k.rec2, k.rec2 + n, k.rec2 + 2 * n};
     const int cshift = p.bv_log + p.fs;
+    int rc = tg::OK;
 
-    hipLaunchKernelGGL(k_l1_count, dim3(p.nchunks), dim3(L1_THREADS), 0, s, tv4, T, p.tpc,
-                       cshift, p.nc, p.nchunks, k.h1);
-    TG_LAUNCH_CHECK("k_l1_count");
-    int rc = tg::scan_exclusive_i32(k.h1, k.h1, (int64_t)p.nc * p.nchunks, b.scan_tmp, s);
-    if (rc) return rc;
-    if (mk) mk->mark("l1_count", s);
-    hipLaunchKernelGGL(k_l1_scatter, dim3(p.nchunks), dim3(L1_THREADS), 0, s, tv4, T, p.tpc,
-                       cshift, p.nc, p.nchunks, k.h1, r1);
-    TG_LAUNCH_CHECK("k_l1_scatter");
-    if (mk) mk->mark("l1_scatter", s);
-    const dim3 g2(p.nc, p.slices);
-    hipLaunchKernelGGL(k_l2_count, g2, dim3(L2_THREADS), 0, s, r1.w0, k.h1, p.nchunks, p.bv_log,
-                       p.fs, p.nb, k.h2);
-    TG_LAUNCH_CHECK("k_l2_count");
-    if ((rc = tg::scan_exclusive_i32(k.h2, k.h2, (int64_t)p.nb * p.slices, b.scan_tmp, s)))
-        return rc;
-    if (mk) mk->mark("l2_count", s);
-    hipLaunchKernelGGL(k_l2_scatter, g2, dim3(L2_THREADS), 0, s, r1, k.h1, p.nchunks, p.bv_log,
-                       p.fs, p.nb, k.h2, r2);
-    TG_LAUNCH_CHECK("k_l2_scatter");
-    if (mk) mk->mark("l2_scatter", s);
+    // routing: one level (bucket ranges in h2[0..nb]) or two levels (h2 is
+    // [bucket][slice]); the choice is made on the mesh's first run
+    int route = k.route ? *k.route : tg::ROUTE_TWO;
+    const int32_t *ranges = k.h2;
+    int32_t nsl = p.slices;
+    if (route != tg::ROUTE_TWO) {
+        int32_t *bstart = k.h2, *bused = k.h1;
+        if ((rc = tg::fill_i32(bstart, 0, p.nb, s)) || (rc = tg::fill_i32(bused, 0, p.nb, s)))
+            return rc;
+        hipLaunchKernelGGL(k_s1_count, dim3(p.nchunks), dim3(S1_THREADS), 0, s, tv4, T, p.tpc,
+                           p.bv_log, bstart, b.counters);
+        TG_LAUNCH_CHECK("k_s1_count");
+        if (route == tg::ROUTE_AUTO) {
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            TG_HIP(hipStreamIsCapturing(s, &cs));
+            if (cs != hipStreamCaptureStatusNone) {
+                route = tg::ROUTE_TWO;  // no waiting under capture; decide on an eager run
+            } else {
+                int32_t wide = 0;
+                TG_HIP(hipMemcpyAsync(&wide, b.counters + tg::C_WIDE, sizeof wide,
+                                      hipMemcpyDeviceToHost, s));
+                TG_HIP(hipStreamSynchronize(s));
+                const int64_t per = (p.tpc + S1_TILE - 1) / S1_TILE;
+                const int64_t last = T - (int64_t)(p.nchunks - 1) * p.tpc;
+                const int64_t tiles = (int64_t)(p.nchunks - 1) * per + (last + S1_TILE - 1) / S1_TILE;
+                route = (int64_t)wide * 64 <= tiles ? tg::ROUTE_ONE : tg::ROUTE_TWO;
+                *k.route = route;
+            }
+        }
+        if (route == tg::ROUTE_ONE) {
+            if ((rc = tg::scan_exclusive_i32(bstart, bstart, p.nb, b.scan_tmp, s))) return rc;
+            if (mk) mk->mark("s1_count", s);
+            hipLaunchKernelGGL(k_s1_scatter, dim3(p.nchunks), dim3(S1_THREADS), 0, s, tv4, T,
+                               p.tpc, p.bv_log, bstart, bused, r2);
+            TG_LAUNCH_CHECK("k_s1_scatter");
+            if (mk) mk->mark("s1_scatter", s);
+            ranges = bstart;
+            nsl = 1;
+        }
+    }
+    if (route == tg::ROUTE_TWO) {
+        hipLaunchKernelGGL(k_l1_count, dim3(p.nchunks), dim3(L1_THREADS), 0, s, tv4, T, p.tpc,
+                           cshift, p.nc, p.nchunks, k.h1);
+        TG_LAUNCH_CHECK("k_l1_count");
+        if ((rc = tg::scan_exclusive_i32(k.h1, k.h1, (int64_t)p.nc * p.nchunks, b.scan_tmp, s)))
+            return rc;
+        if (mk) mk->mark("l1_count", s);
+        hipLaunchKernelGGL(k_l1_scatter, dim3(p.nchunks), dim3(L1_THREADS), 0, s, tv4, T, p.tpc,
+                           cshift, p.nc, p.nchunks, k.h1, r1);
+        TG_LAUNCH_CHECK("k_l1_scatter");
+        if (mk) mk->mark("l1_scatter", s);
+        const dim3 g2(p.nc, p.slices);
+        hipLaunchKernelGGL(k_l2_count, g2, dim3(L2_THREADS), 0, s, r1.w0, k.h1, p.nchunks,
+                           p.bv_log, p.fs, p.nb, k.h2);
+        TG_LAUNCH_CHECK("k_l2_count");
+        if ((rc = tg::scan_exclusive_i32(k.h2, k.h2, (int64_t)p.nb * p.slices, b.scan_tmp, s)))
+            return rc;
+        if (mk) mk->mark("l2_count", s);
+        hipLaunchKernelGGL(k_l2_scatter, g2, dim3(L2_THREADS), 0, s, r1, k.h1, p.nchunks,
+                           p.bv_log, p.fs, p.nb, k.h2, r2);
+        TG_LAUNCH_CHECK("k_l2_scatter");
+        if (mk) mk->mark("l2_scatter", s);
+    }
 #define TG_BKT(KERNEL, BVL)                                                                  \
-    hipLaunchKernelGGL((KERNEL<BVL, S>), dim3(p.nb), dim3(2 << BVL), 0, s, r2, k.h2, p.slices, \
+    hipLaunchKernelGGL((KERNEL<BVL, S>), dim3(p.nb), dim3(2 << BVL), 0, s, r2, ranges, nsl,   \
                        scal, V, cls, b.scratch, b.deg, k.rowsrc, b.classes, b.heavy, b.fixd,   \
                        b.rowdesc, b.counters)
     if (p.owner) {
@@ -1022,11 +1248,11 @@ int launch_bucketed_typed(tg::BucketBuffers &k, bool classify, hipStream_t s) {
     TG_LAUNCH_CHECK("k_bkt_vv_cp");
     if (mk) mk->mark("bkt_vv_cp", s);
     if (p.bv_log == 7)
-        hipLaunchKernelGGL((k_bkt_heavy<7, S>), dim3(256), dim3(64), 0, s, r2, k.h2, p.slices,
+        hipLaunchKernelGGL((k_bkt_heavy<7, S>), dim3(256), dim3(64), 0, s, r2, ranges, nsl,
                            scal, cls, b.scratch, b.deg, k.rowsrc, b.classes, b.heavy, b.fix,
                            b.fixbig, b.counters);
     else
-        hipLaunchKernelGGL((k_bkt_heavy<8, S>), dim3(256), dim3(64), 0, s, r2, k.h2, p.slices,
+        hipLaunchKernelGGL((k_bkt_heavy<8, S>), dim3(256), dim3(64), 0, s, r2, ranges, nsl,
                            scal, cls, b.scratch, b.deg, k.rowsrc, b.classes, b.heavy, b.fix,
                            b.fixbig, b.counters);
     TG_LAUNCH_CHECK("k_bkt_heavy");

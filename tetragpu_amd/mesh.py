@@ -113,6 +113,19 @@ class Mesh:
     def set_config(self, lane_cap: int = 32, lane_block: int = 128, variant: int = 0) -> None:
         check(self._lib.tg_mesh_set_config(self._h, variant, lane_cap, lane_block))
 
+    def set_bucket(self, bv_log: int = 7, tets_per_chunk: int = 8192) -> None:
+        """Bucketed form: 2^bv_log vertices per bucket, tets per routing chunk."""
+        check(self._lib.tg_mesh_set_bucket(self._h, bv_log, tets_per_chunk))
+
+    def set_routing(self, route: int = 0) -> None:
+        """Bucketed form's record routing: 0 auto, 1 one level, 2 two levels."""
+        check(self._lib.tg_mesh_set_routing(self._h, route))
+
+    def routing(self) -> int:
+        r = C.c_int32()
+        check(self._lib.tg_mesh_routing(self._h, C.byref(r)))
+        return r.value
+
     def build(self, mask: int) -> None:
         check(self._lib.tg_build(self._h, mask))
 
