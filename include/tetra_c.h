@@ -102,12 +102,12 @@ int tg_mesh_set_config(tg_mesh *m, int32_t variant, int32_t lane_cap,
  * it.  Chosen by the autotuner per size class. */
 int tg_mesh_set_bucket(tg_mesh *m, int32_t bv_log, int64_t tets_per_chunk);
 
-/* Routing of the bucketed form's corner records: 0 automatic (decided on
- * the mesh's first run: one level when the corners of 1024 consecutive tets
- * span fewer than 1024 buckets in at least 63 of 64 such tiles, else two
- * levels), 1 one level (tiles -> buckets), 2 two levels (chunks -> coarse
- * bins -> buckets).  Results do not depend on it.  tg_mesh_routing reports
- * the routing in use (0 until the first bucketed run in automatic mode). */
+/* Routing of the bucketed form's corner records: 0 automatic (one level
+ * when the corners of 1024 consecutive tets span fewer than 1024 buckets in
+ * at least 63 of 64 such tiles, else two levels; the spans are measured by
+ * the connectivity check of tg_mesh_create), 1 one level (tiles ->
+ * buckets), 2 two levels (chunks -> coarse bins -> buckets).  Results do
+ * not depend on it.  tg_mesh_routing reports the routing in use. */
 int tg_mesh_set_routing(tg_mesh *m, int32_t route);
 int tg_mesh_routing(tg_mesh *m, int32_t *route);
 

@@ -58,16 +58,52 @@ __global__ void k_convert(const A *__restrict__ in, int64_t n, B *__restrict__ o
         out[i] = (B)in[i];
 }
 
-__global__ void k_validate(const int4 *__restrict__ tv4, int64_t T, int64_t V,
-                           int32_t *__restrict__ err) {
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < T;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        const int4 q = tv4[t];
-        const bool range = q.x >= 0 && q.y >= 0 && q.z >= 0 && q.w >= 0 && q.x < V &&
-                           q.y < V && q.z < V && q.w < V;
-        const bool distinct = q.x != q.y && q.x != q.z && q.x != q.w && q.y != q.z &&
-                              q.y != q.w && q.z != q.w;
-        if (!(range && distinct)) atomicOr(err, 1);
+// Connectivity check (err[0] |= 1 for an id outside [0, V) or a vertex
+// repeated in a cell) and, per tile of 1024 consecutive cells, whether its
+// vertices span 1024 or more buckets of 2^7 (err[1] += 1) or 2^8 (err[2] += 1)
+// vertices: the input to the bucketed form's routing choice.
+constexpr int VAL_TILE = 1024, VAL_THREADS = 256;
+__global__ __launch_bounds__(VAL_THREADS) void k_validate(const int4 *__restrict__ tv4, int64_t T,
+                                                          int64_t V, int32_t *__restrict__ err) {
+    __shared__ int32_t red[2][VAL_THREADS / 64];
+    const int64_t ntiles = (T + VAL_TILE - 1) / VAL_TILE;
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        int32_t lo = INT32_MAX, hi = -1;
+        bool bad = false;
+        for (int k = 0; k < VAL_TILE / VAL_THREADS; k++) {
+            const int64_t t = tile * VAL_TILE + k * VAL_THREADS + threadIdx.x;
+            if (t >= T) break;
+            const int4 q = tv4[t];
+            const bool range = q.x >= 0 && q.y >= 0 && q.z >= 0 && q.w >= 0 && q.x < V &&
+                               q.y < V && q.z < V && q.w < V;
+            const bool distinct = q.x != q.y && q.x != q.z && q.x != q.w && q.y != q.z &&
+                                  q.y != q.w && q.z != q.w;
+            bad |= !(range && distinct);
+            lo = min(lo, min(min(q.x, q.y), min(q.z, q.w)));
+            hi = max(hi, max(max(q.x, q.y), max(q.z, q.w)));
+        }
+        if (bad) atomicOr(err, 1);
+#pragma unroll
+        for (int s = 32; s >= 1; s >>= 1) {
+            lo = min(lo, __shfl_xor(lo, s, 64));
+            hi = max(hi, __shfl_xor(hi, s, 64));
+        }
+        if (__lane_id() == 0) {
+            red[0][threadIdx.x / 64] = lo;
+            red[1][threadIdx.x / 64] = hi;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 1; w < VAL_THREADS / 64; w++) {
+                lo = min(lo, red[0][w]);
+                hi = max(hi, red[1][w]);
+            }
+            if (lo >= 0 && hi >= lo) {
+                if ((hi >> 7) - (lo >> 7) >= 1024) atomicAdd(err + 1, 1);
+                if ((hi >> 8) - (lo >> 8) >= 1024) atomicAdd(err + 2, 1);
+            }
+        }
+        __syncthreads();  // red is reused by the next tile
     }
 }
 
@@ -180,15 +216,20 @@ int tg_mesh_create(tg_mesh **out, const int32_t *cells, int64_t n_cells,
         if (e != hipSuccess) return bail(tg::hip_fail(e, "hipMemcpyAsync(cells)", __FILE__, __LINE__));
         // validate connectivity once
         int32_t *err = m->counters.as<int32_t>();
-        if ((rc = tg::fill_i32(err, 0, 1, m->stream))) return bail(rc);
-        unsigned g = std::min<unsigned>(tg::grid_for(n_cells, 256), 16384u);
-        hipLaunchKernelGGL(k_validate, dim3(g), dim3(256), 0, m->stream,
+        if ((rc = tg::fill_i32(err, 0, 3, m->stream))) return bail(rc);
+        const int64_t tiles = (n_cells + VAL_TILE - 1) / VAL_TILE;
+        const unsigned g = (unsigned)std::min<int64_t>(tiles, 16384);
+        hipLaunchKernelGGL(k_validate, dim3(g), dim3(VAL_THREADS), 0, m->stream,
                            (const int4 *)m->tv.p, n_cells, n_points, err);
         if (hipGetLastError() != hipSuccess) return bail(fail(TG_ERR_HIP, "k_validate launch"));
-        int32_t herr = 0;
-        if (hipMemcpyAsync(&herr, err, sizeof herr, hipMemcpyDeviceToHost, m->stream) != hipSuccess ||
+        int32_t hv[3] = {0, 0, 0};
+        if (hipMemcpyAsync(hv, err, sizeof hv, hipMemcpyDeviceToHost, m->stream) != hipSuccess ||
             hipStreamSynchronize(m->stream) != hipSuccess)
             return bail(fail(TG_ERR_HIP, "validation readback failed"));
+        m->span_tiles = tiles;
+        m->span_wide[0] = hv[1];
+        m->span_wide[1] = hv[2];
+        const int32_t herr = hv[0];
         if (herr) return bail(fail(TG_ERR_MESH, "cells hold a vertex id outside [0, n_points) "
                                                  "or a vertex repeated within a cell"));
     }

@@ -45,6 +45,10 @@ struct tg_mesh {
     tg::DevArr bk_h1, bk_h2, bk_rowsrc, bk_fixd, bk_rowdesc;
     tg::BucketPlan bplan;
     int32_t bk_route = 0;  // tg::Route of the bucketed pipeline (AUTO until the first run)
+    // measured at creation: tiles of 1024 cells, and those whose vertices span
+    // >= 1024 buckets of 2^7 / 2^8 vertices (span_tiles < 0: not measured)
+    int64_t span_tiles = -1;
+    int32_t span_wide[2] = {0, 0};
     bool vt_ready = false, bk_ready = false;
     bool hot_ready = false;         // buffers sized for the hot path
     bool vt_corners_built = false;  // vt_off / vtc valid

@@ -110,6 +110,10 @@ int prepare_hot_path(tg_mesh &m) {
     resolve_tuning(m);
     m.bplan = plan_buckets(m.n_cells, m.n_points, m.cfg.bv_log, m.cfg.tpc);
     m.bk_route = m.cfg.route;
+    if (m.bk_route == ROUTE_AUTO && m.span_tiles >= 0)  // decided from the creation-time spans
+        m.bk_route = (int64_t)m.span_wide[m.cfg.bv_log == 8 ? 1 : 0] * 64 <= m.span_tiles
+                         ? ROUTE_ONE
+                         : ROUTE_TWO;
     m.hot_ready = true;
     return OK;
 }
