@@ -195,6 +195,65 @@ def bench_serve(t, rates, n_requests: int, seed: int = 11) -> dict:
     return out
 
 
+def bench_irregular(t, torch, steps: int, seed: int = 5, check: bool = True) -> dict:
+    """Kuhn 96^3 with 15 % of the tets stellar-split 1->4 at their centroid,
+    plus 32 hub vertices each raised by 64 repeated splits around them, so
+    that the largest VV degree exceeds 60 (seeded vertex/tet permutation, 8
+    sinusoidal modes): VV + critical points per step, then TT and EE built
+    once on the resident mesh.  With check, VV and classes are compared with
+    the OpenMP oracle on the full mesh (bit-exact)."""
+    import numpy as np
+
+    m = t.kuhn(96, seed=seed, split_fraction=0.15, n_hubs=32, hub_splits=64, scalar="sines")
+    T, V = m.n_cells, m.n_points
+    log(f"irregular: {T} tets, {V} vertices")
+    stream = torch.cuda.Stream()
+    out = {"workload": "kuhn96_split15_hubs32x64_sines", "tets": T, "vertices": V}
+    with t.Mesh(m.cells, V, m.scalars, stream=stream.cuda_stream) as d:
+        for _ in range(2):
+            d.vv_critpoints()
+        d.sync()
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(steps):
+            d.vv_critpoints()
+        b.record(stream)
+        torch.cuda.synchronize()
+        d.sync()
+        ms = a.elapsed_time(b) / steps
+        d.profile(True)
+        d.vv_critpoints()
+        out["stages_ms"] = {k: round(v, 4) for k, v in d.stage_times()}
+        d.profile(False)
+        off, vals = d.relation("VV")
+        deg = np.diff(off)
+        cls = d.classes()
+        out.update({"ms_per_step": round(ms, 4), "tets_per_s": round(T / (ms * 1e-3), 1),
+                    "vv_nnz": int(off[-1]), "max_vv_degree": int(deg.max()),
+                    "vertices_over_32": int((deg > 32).sum()),
+                    "vertices_over_64": int((deg > 64).sum())})
+        for name in ("TT", "EE"):
+            torch.cuda.synchronize()
+            a.record(stream)
+            d.build(t.flags(name))
+            b.record(stream)
+            torch.cuda.synchronize()
+            d.sync()
+            out[f"{name.lower()}_build_ms"] = round(a.elapsed_time(b), 3)
+            out[f"{name.lower()}_nnz"] = d.shape(name)["nnz"]
+    if check:
+        from oracle import binding as O
+
+        VV, ref, secs = O.vv_cp_omp(m.cells, V, m.scalars, cpu_threads())
+        same_vv = bool(np.array_equal(VV.off, off) and np.array_equal(VV.val[:, 0], vals[:, 0]))
+        out["identical_to_oracle"] = bool(same_vv and np.array_equal(ref, cls))
+        out["oracle_s"] = round(secs, 2)
+    log(f"irregular: {out.get('ms_per_step')} ms per step, identical {out.get('identical_to_oracle')}")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -208,6 +267,8 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=60,
                     help="requests per arrival rate in the serving measurement (0: skip)")
     ap.add_argument("--rates", type=float, nargs="+", default=[100, 500, 1000, 2000, 5000])
+    ap.add_argument("--no-irregular", action="store_true",
+                    help="skip the irregular-degree mesh (Kuhn 96^3, 15 %% split)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (reference kernel for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -313,6 +374,8 @@ def main() -> int:
     e2e_ms = (time.perf_counter() - t0) * 1e3
     d2.close()
 
+    irregular = None if args.no_irregular else bench_irregular(t, torch, max(args.steps // 2, 1),
+                                                                check=not args.no_cpu)
     batch = bench_batch(t, torch, args.batch_meshes, 3) if args.batch_meshes > 0 else None
     serve = bench_serve(t, args.rates, args.serve_requests) if args.serve_requests > 0 else None
 
@@ -336,6 +399,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "end_to_end_ms": round(e2e_ms, 2),
         "end_to_end_note": "host cells/scalars in, VV CSR + classes out, incl. allocation (first use)",
+        "irregular": irregular,
         "batched": batch,
         "serving": serve,
     }

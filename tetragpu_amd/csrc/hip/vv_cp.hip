@@ -389,70 +389,8 @@ __global__ __launch_bounds__(256) void k_rows_compact(
 }
 
 // ---- row-membership link components for flagged vertices -------------------
-// Link edge (a, c) between two neighbours of v exists iff c is in VV(a)
-// (critPoints.cu:231-233).  Each undirected edge is found once, from the
-// smaller endpoint, by merging the ascending rows VV(a) (entries > a) and
-// VV(v) (entries after a).
-template <typename S>
-__device__ void link_components(int32_t v, const int32_t *__restrict__ off,
-                                const int32_t *__restrict__ cols,
-                                const S *__restrict__ scal, int d,
-                                const int32_t *nb, uint16_t *par, int pstride,
-                                uint8_t *lowf, int lstride, uint32_t &nl,
-                                uint32_t &nu) {
-    const S sv = scal[v];
-    for (int i = 0; i < d; i++) {
-        const int32_t a = nb[i];
-        par[i * pstride] = (uint16_t)i;
-        lowf[i * lstride] = (uint8_t)is_lower(scal[a], sv, a, v);
-    }
-    auto root = [&](int x) {
-        int p;
-        while ((p = par[x * pstride]) != x) {
-            const int g = par[p * pstride];
-            par[x * pstride] = (uint16_t)g;
-            x = g;
-        }
-        return x;
-    };
-    for (int ia = 0; ia < d - 1; ia++) {
-        const int32_t a = nb[ia];
-        int32_t j = off[a];
-        const int32_t je = off[a + 1];
-        // first entry of VV(a) greater than a
-        int32_t lo = j, hi = je;
-        while (lo < hi) {
-            const int32_t mid = (lo + hi) >> 1;
-            if (cols[mid] <= a) lo = mid + 1;
-            else hi = mid;
-        }
-        j = lo;
-        int ic = ia + 1;
-        const uint8_t la = lowf[ia * lstride];
-        while (j < je && ic < d) {
-            const int32_t c = cols[j], w = nb[ic];
-            if (c < w) j++;
-            else if (w < c) ic++;
-            else {
-                if (lowf[ic * lstride] == la) {
-                    int ra = root(ia), rc = root(ic);
-                    if (ra != rc) {
-                        if (ra < rc) par[rc * pstride] = (uint16_t)ra;
-                        else par[ra * pstride] = (uint16_t)rc;
-                    }
-                }
-                j++;
-                ic++;
-            }
-        }
-    }
-    nl = nu = 0;
-    for (int i = 0; i < d; i++)
-        if (par[i * pstride] == i) {
-            if (lowf[i * lstride]) nl++;
-            else nu++;
-        }
-}
+// Link edge (a, c) between two neighbours a < c of v exists iff c is in
+// VV(a) (critPoints.cu:231-233).
 
 __device__ __forceinline__ uint32_t n_components32(const uint32_t *adj, uint32_t mask) {
     uint32_t cnt = 0, rem = mask;
@@ -646,28 +584,92 @@ __global__ __launch_bounds__(64) void k_cp_fix_desc(
     }
 }
 
+// Row-membership link pass for vertices with more than 64 neighbours: one
+// workgroup per vertex.  Each thread walks the rows of some neighbours a and
+// looks every later entry c up in v's row (binary search in LDS); pairs of
+// one class are joined in a lock-free union-find (a root is only linked
+// under a smaller root, by CAS).  Same edge set as link_components.
+constexpr int FB_THREADS = 256;
+
+__device__ __forceinline__ int fb_root(int32_t *par, int x) {
+    while (true) {
+        const int p = par[x];
+        if (p == x) return x;
+        const int g = par[p];
+        if (g != p) par[x] = g;
+        x = g;
+    }
+}
+
+__device__ __forceinline__ void fb_unite(int32_t *par, int a, int b) {
+    while (true) {
+        a = fb_root(par, a);
+        b = fb_root(par, b);
+        if (a == b) return;
+        if (a > b) {
+            const int t = a;
+            a = b;
+            b = t;
+        }
+        if (atomicCAS(&par[b], b, a) == b) return;
+    }
+}
+
 template <typename S>
-__global__ __launch_bounds__(64) void k_cp_fix_big(
+__global__ __launch_bounds__(FB_THREADS) void k_cp_fix_big(
     const int32_t *__restrict__ off, const int32_t *__restrict__ cols,
     const S *__restrict__ scal, const int32_t *__restrict__ fixbig,
     int32_t *__restrict__ counters, uint32_t *__restrict__ classes) {
-    __shared__ uint16_t par[HEAVY_CAP];
+    __shared__ int32_t nb[HEAVY_CAP];
+    __shared__ int32_t par[HEAVY_CAP];
     __shared__ uint8_t lowf[HEAVY_CAP];
-    if (threadIdx.x != 0) return;
+    __shared__ int32_t cnt[2];
+    const int tid = threadIdx.x;
     const int32_t n = counters[C_FIXBIG];
     for (int32_t i = blockIdx.x; i < n; i += gridDim.x) {
         const int32_t v = fixbig[i];
         const int32_t b = off[v];
         const int d = off[v + 1] - b;
-        if (d > HEAVY_CAP) {
-            atomicOr(counters + C_ERR, tg::ERRBIT_CAPACITY);
+        if (d > HEAVY_CAP) {  // block-uniform
+            if (tid == 0) atomicOr(counters + C_ERR, tg::ERRBIT_CAPACITY);
             continue;
         }
-        uint32_t nl, nu;
-        link_components(v, off, cols, scal, d, cols + b, par, 1, lowf, 1, nl, nu);
-        classes[3 * (int64_t)v] = nl;
-        classes[3 * (int64_t)v + 1] = nu;
-        classes[3 * (int64_t)v + 2] = verdict(nl, nu);
+        const S sv = scal[v];
+        for (int k = tid; k < d; k += FB_THREADS) {
+            const int32_t a = cols[b + k];
+            nb[k] = a;
+            par[k] = k;
+            lowf[k] = (uint8_t)is_lower(scal[a], sv, a, v);
+        }
+        if (tid < 2) cnt[tid] = 0;
+        __syncthreads();
+        for (int ia = tid; ia < d; ia += FB_THREADS) {
+            const int32_t a = nb[ia];
+            const uint8_t la = lowf[ia];
+            const int32_t je = off[a + 1];
+            for (int32_t j = off[a]; j < je; j++) {
+                const int32_t c = cols[j];
+                if (c <= a) continue;
+                int lo = 0, hi = d;  // first entry of v's row >= c
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (nb[mid] < c) lo = mid + 1;
+                    else hi = mid;
+                }
+                if (lo < d && nb[lo] == c && lowf[lo] == la) fb_unite(par, ia, lo);
+            }
+        }
+        __syncthreads();
+        for (int k = tid; k < d; k += FB_THREADS)
+            if (par[k] == k) atomicAdd(&cnt[lowf[k] ? 0 : 1], 1);
+        __syncthreads();
+        if (tid == 0) {
+            const uint32_t nl = (uint32_t)cnt[0], nu = (uint32_t)cnt[1];
+            classes[3 * (int64_t)v] = nl;
+            classes[3 * (int64_t)v + 1] = nu;
+            classes[3 * (int64_t)v + 2] = verdict(nl, nu);
+        }
+        __syncthreads();  // nb, par, lowf and cnt are reused by the next vertex
     }
 }
 
@@ -685,7 +687,7 @@ int launch_cp_fix_typed(tg::VvCpBuffers &b, hipStream_t s) {
     hipLaunchKernelGGL((k_cp_fix_wave<S>), dim3(g), dim3(64), 0, s, b.vv_off, b.vv_cols, scal,
                        b.fix, b.fixbig, b.counters, b.classes);
     TG_LAUNCH_CHECK("k_cp_fix_wave");
-    hipLaunchKernelGGL((k_cp_fix_big<S>), dim3(256), dim3(64), 0, s, b.vv_off, b.vv_cols, scal,
+    hipLaunchKernelGGL((k_cp_fix_big<S>), dim3(256), dim3(FB_THREADS), 0, s, b.vv_off, b.vv_cols, scal,
                        b.fixbig, b.counters, b.classes);
     TG_LAUNCH_CHECK("k_cp_fix_big");
     if (b.marks) b.marks->mark("cp_fix", s);

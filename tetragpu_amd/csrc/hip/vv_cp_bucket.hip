@@ -19,7 +19,8 @@
 //       from the shared-tet link triangles, lower/upper masks from the
 //       scalars (ballots), link components by bit-parallel BFS, rows sorted
 //       by a 32-register sorting network and staged in LDS for coalesced
-//       stores.  Vertices with more than 32 neighbours go to k_bkt_heavy.
+//       stores.  Vertices with more than 32 neighbours go to k_bkt_heavy
+//       (one workgroup per vertex: LDS hash set, lock-free union-find).
 // Then: scan of degrees, row compaction into the CSR, and the row-membership
 // link pass for vertices whose shared-tet link has >= 2 components in a class
 // (the shared-tet link graph is a subgraph of the reference's, so counts of 0
@@ -1051,106 +1052,185 @@ __global__ __launch_bounds__(2 << BV_LOG) void k_bkt_own(
     }
 }
 
-// Serial fallback for vertices with more than 32 neighbours: lane 0 scans
-// its bucket's corner records, keeping a 4096-entry neighbour list in LDS.
+// Vertices with more than 32 neighbours (or whose bucket table overflowed):
+// one workgroup per vertex scans its bucket's corner records.  Neighbours go
+// to an LDS open-addressing set of 2 * HEAVY_CAP slots (CAS inserts), the
+// shared-tet link edges between same-class slots to a lock-free union-find
+// (a root is only ever linked under a smaller root, by CAS), and the row is
+// sorted by a bitonic network in LDS.
+constexpr int HV_THREADS = 256;
+constexpr int HV_SLOTS = 2 * HEAVY_CAP;
+constexpr int HV_BITS = 13;  // log2(HV_SLOTS)
+static_assert((1 << HV_BITS) == HV_SLOTS, "heavy set size");
+
+// Slot of u in the set (inserted when absent), or -1 when the set is full.
+__device__ __forceinline__ int hv_insert(int32_t *set, int32_t u, int32_t *count) {
+    uint32_t i = ((uint32_t)u * 2654435769u) >> (32 - HV_BITS);
+    for (int probe = 0; probe < HV_SLOTS; probe++, i = (i + 1) & (HV_SLOTS - 1)) {
+        const int32_t k = set[i];
+        if (k == u) return (int)i;
+        if (k == -1) {
+            const int32_t old = atomicCAS(&set[i], -1, u);
+            if (old == -1) {
+                atomicAdd(count, 1);
+                return (int)i;
+            }
+            if (old == u) return (int)i;
+        }
+    }
+    return -1;
+}
+
+// Slot of u, which is in the set.
+__device__ __forceinline__ int hv_find(const int32_t *set, int32_t u) {
+    uint32_t i = ((uint32_t)u * 2654435769u) >> (32 - HV_BITS);
+    for (int probe = 0; probe < HV_SLOTS; probe++, i = (i + 1) & (HV_SLOTS - 1))
+        if (set[i] == u) return (int)i;
+    return -1;
+}
+
+// Lock-free union-find over par[] (par[x] <= x always; path halving).
+__device__ __forceinline__ int uf_root(int32_t *par, int x) {
+    while (true) {
+        const int p = par[x];
+        if (p == x) return x;
+        const int g = par[p];
+        if (g != p) par[x] = g;
+        x = g;
+    }
+}
+
+__device__ __forceinline__ void uf_unite(int32_t *par, int a, int b) {
+    while (true) {
+        a = uf_root(par, a);
+        b = uf_root(par, b);
+        if (a == b) return;
+        if (a > b) {
+            const int t = a;
+            a = b;
+            b = t;
+        }
+        if (atomicCAS(&par[b], b, a) == b) return;
+    }
+}
+
+// Ascending bitonic sort of list[0..P) in LDS (P a power of two), whole block.
+__device__ __forceinline__ void block_bitonic(int32_t *list, int P) {
+    for (int k = 2; k <= P; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < P; i += blockDim.x) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const int32_t a = list[i], b = list[l];
+                    const bool up = (i & k) == 0;
+                    if ((a > b) == up) {
+                        list[i] = b;
+                        list[l] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+}
+
 template <int BV_LOG, typename S>
-__global__ __launch_bounds__(64) void k_bkt_heavy(
+__global__ __launch_bounds__(HV_THREADS) void k_bkt_heavy(
     const Rec rec, const int32_t *__restrict__ h2s, int32_t nslice,
     const S *__restrict__ scal, int classify, int32_t *__restrict__ scratch,
     int32_t *__restrict__ deg, const int32_t *__restrict__ rowsrc,
     uint32_t *__restrict__ classes, const int32_t *__restrict__ heavy,
     int32_t *__restrict__ fix, int32_t *__restrict__ fixbig, int32_t *__restrict__ counters) {
+    __shared__ int32_t set[HV_SLOTS];
+    __shared__ int32_t par[HV_SLOTS];
+    __shared__ uint8_t lowf[HV_SLOTS];
     __shared__ int32_t list[HEAVY_CAP];
-    __shared__ uint16_t par[HEAVY_CAP];
-    __shared__ uint8_t lowf[HEAVY_CAP];
-    if (threadIdx.x != 0) return;
+    __shared__ int32_t misc[5];  // distinct neighbours, full flag, lower roots, upper roots, fill
+    const int tid = threadIdx.x;
     const int32_t n = counters[C_HEAVY];
     for (int32_t h = blockIdx.x; h < n; h += gridDim.x) {
         const int32_t v = heavy[h];
         const int32_t bkt = v >> BV_LOG;
         const int32_t cbeg = h2s[(int64_t)bkt * nslice], cend = h2s[(int64_t)(bkt + 1) * nslice];
         const S sv = classify ? scal[v] : S(0);
-        int m = 0;
-        bool over = false;
-        auto find_or_insert = [&](int32_t u) -> int {
-            for (int i = 0; i < m; i++)
-                if (list[i] == u) return i;
-            if (m == HEAVY_CAP) return -1;
-            list[m] = u;
-            par[m] = (uint16_t)m;
-            lowf[m] = classify ? (uint8_t)is_lower(scal[u], sv, u, v) : 0;
-            return m++;
-        };
-        auto root = [&](int x) {
-            while (par[x] != x) {
-                par[x] = par[par[x]];
-                x = par[x];
-            }
-            return x;
-        };
-        auto unite = [&](int a, int b) {
-            if (lowf[a] != lowf[b]) return;
-            a = root(a);
-            b = root(b);
-            if (a == b) return;
-            if (a < b) par[b] = (uint16_t)a;
-            else par[a] = (uint16_t)b;
-        };
-        for (int32_t i = cbeg; i < cend && !over; i++) {
+        for (int i = tid; i < HV_SLOTS; i += HV_THREADS) {
+            set[i] = -1;
+            par[i] = i;
+        }
+        if (tid < 5) misc[tid] = 0;
+        __syncthreads();
+        // neighbour set
+        for (int32_t i = cbeg + tid; i < cend; i += HV_THREADS) {
             uint32_t rv, x, y, z;
             rec_unpack(rec.w0[i], rec.w1[i], rec.w2[i], rv, x, y, z);
             if ((int32_t)rv != v) continue;
-            const int i0 = find_or_insert((int32_t)x);
-            const int i1 = find_or_insert((int32_t)y);
-            const int i2 = find_or_insert((int32_t)z);
-            if ((i0 | i1 | i2) < 0) {
-                over = true;
-                break;
-            }
-            if (classify) {
-                unite(i0, i1);
-                unite(i0, i2);
-                unite(i1, i2);
-            }
+            if ((hv_insert(set, (int32_t)x, &misc[0]) | hv_insert(set, (int32_t)y, &misc[0]) |
+                 hv_insert(set, (int32_t)z, &misc[0])) < 0)
+                misc[1] = 1;
         }
-        if (over) {
-            atomicOr(counters + C_ERR, tg::ERRBIT_CAPACITY);
-            deg[v] = 0;
-            if (classify) {
-                classes[3 * (int64_t)v] = 0;
-                classes[3 * (int64_t)v + 1] = 0;
-                classes[3 * (int64_t)v + 2] = 0;
+        __syncthreads();
+        const int m = misc[0];
+        if (misc[1] || m > HEAVY_CAP) {  // block-uniform
+            if (tid == 0) {
+                atomicOr(counters + C_ERR, tg::ERRBIT_CAPACITY);
+                deg[v] = 0;
+                if (classify) {
+                    classes[3 * (int64_t)v] = 0;
+                    classes[3 * (int64_t)v + 1] = 0;
+                    classes[3 * (int64_t)v + 2] = 0;
+                }
             }
+            __syncthreads();
             continue;
         }
-        uint32_t nl = 0, nu = 0;
         if (classify) {
-            for (int i = 0; i < m; i++)
-                if (par[i] == i) {
-                    if (lowf[i]) nl++;
-                    else nu++;
-                }
-            classes[3 * (int64_t)v] = nl;
-            classes[3 * (int64_t)v + 1] = nu;
-            classes[3 * (int64_t)v + 2] = verdict(nl, nu);
-        }
-        deg[v] = m;
-        atomicMax(counters + C_MAXDEG, m);
-        for (int i = 1; i < m; i++) {
-            const int32_t x = list[i];
-            int j = i - 1;
-            while (j >= 0 && list[j] > x) {
-                list[j + 1] = list[j];
-                j--;
+            for (int i = tid; i < HV_SLOTS; i += HV_THREADS) {
+                const int32_t u = set[i];
+                lowf[i] = u >= 0 ? (uint8_t)is_lower(scal[u], sv, u, v) : (uint8_t)0;
             }
-            list[j + 1] = x;
+            __syncthreads();
+            // shared-tet link edges between neighbours of one class
+            for (int32_t i = cbeg + tid; i < cend; i += HV_THREADS) {
+                uint32_t rv, x, y, z;
+                rec_unpack(rec.w0[i], rec.w1[i], rec.w2[i], rv, x, y, z);
+                if ((int32_t)rv != v) continue;
+                const int a = hv_find(set, (int32_t)x), b = hv_find(set, (int32_t)y),
+                          c = hv_find(set, (int32_t)z);
+                if ((a | b | c) < 0) continue;  // not reached: every key was inserted above
+                if (lowf[a] == lowf[b]) uf_unite(par, a, b);
+                if (lowf[a] == lowf[c]) uf_unite(par, a, c);
+                if (lowf[b] == lowf[c]) uf_unite(par, b, c);
+            }
+            __syncthreads();
         }
+        // row (compacted) and component roots per class
+        for (int i = tid; i < HV_SLOTS; i += HV_THREADS) {
+            const int32_t u = set[i];
+            if (u < 0) continue;
+            list[atomicAdd(&misc[4], 1)] = u;
+            if (classify && par[i] == i) atomicAdd(&misc[lowf[i] ? 2 : 3], 1);
+        }
+        int P = 1;
+        while (P < m) P <<= 1;
+        for (int i = m + tid; i < P; i += HV_THREADS) list[i] = INT32_MAX;
+        __syncthreads();
+        block_bitonic(list, P);
         int32_t *dst = scratch + rowsrc[v];
-        for (int i = 0; i < m; i++) dst[i] = list[i];
-        if (classify && (nl >= 2 || nu >= 2)) {
-            if (m <= 64) fix[atomicAdd(counters + C_FIX, 1)] = v;
-            else fixbig[atomicAdd(counters + C_FIXBIG, 1)] = v;
+        for (int i = tid; i < m; i += HV_THREADS) dst[i] = list[i];
+        if (tid == 0) {
+            const uint32_t nl = (uint32_t)misc[2], nu = (uint32_t)misc[3];
+            deg[v] = m;
+            atomicMax(counters + C_MAXDEG, m);
+            if (classify) {
+                classes[3 * (int64_t)v] = nl;
+                classes[3 * (int64_t)v + 1] = nu;
+                classes[3 * (int64_t)v + 2] = verdict(nl, nu);
+                if (nl >= 2 || nu >= 2) {
+                    if (m <= 64) fix[atomicAdd(counters + C_FIX, 1)] = v;
+                    else fixbig[atomicAdd(counters + C_FIXBIG, 1)] = v;
+                }
+            }
         }
+        __syncthreads();  // the set, list and counters are reused by the next vertex
     }
 }
 
@@ -1248,11 +1328,11 @@ int launch_bucketed_typed(tg::BucketBuffers &k, bool classify, hipStream_t s) {
     TG_LAUNCH_CHECK("k_bkt_vv_cp");
     if (mk) mk->mark("bkt_vv_cp", s);
     if (p.bv_log == 7)
-        hipLaunchKernelGGL((k_bkt_heavy<7, S>), dim3(256), dim3(64), 0, s, r2, ranges, nsl,
+        hipLaunchKernelGGL((k_bkt_heavy<7, S>), dim3(256), dim3(HV_THREADS), 0, s, r2, ranges, nsl,
                            scal, cls, b.scratch, b.deg, k.rowsrc, b.classes, b.heavy, b.fix,
                            b.fixbig, b.counters);
     else
-        hipLaunchKernelGGL((k_bkt_heavy<8, S>), dim3(256), dim3(64), 0, s, r2, ranges, nsl,
+        hipLaunchKernelGGL((k_bkt_heavy<8, S>), dim3(256), dim3(HV_THREADS), 0, s, r2, ranges, nsl,
                            scal, cls, b.scratch, b.deg, k.rowsrc, b.classes, b.heavy, b.fix,
                            b.fixbig, b.counters);
     TG_LAUNCH_CHECK("k_bkt_heavy");
