@@ -195,6 +195,37 @@ def bench_serve(t, rates, n_requests: int, seed: int = 11) -> dict:
     return out
 
 
+def bench_relations(t, torch, mesh) -> dict:
+    """First-build time of each relation on a fresh device mesh (prerequisites
+    included, e.g. edge ids for EE), and of all of them in one request."""
+    names = ["VE", "TE", "EV", "ET", "VF", "TF", "FV", "FE", "FT", "EF", "VT", "TT", "FF",
+             "EE", "VV"]
+    out = {"per_relation_ms": {}}
+
+    def timed(mask):
+        stream = torch.cuda.Stream()
+        with t.Mesh(mesh.cells, mesh.n_points, mesh.scalars, stream=stream.cuda_stream) as d:
+            d.sync()
+            torch.cuda.synchronize()
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            d.build(mask)
+            if mask == t.ALL_FLAGS:
+                d.critpoints()
+            b.record(stream)
+            torch.cuda.synchronize()
+            d.sync()
+            return a.elapsed_time(b)
+
+    for name in names:
+        out["per_relation_ms"][name] = round(timed(t.flags(name)), 3)
+    ms = timed(t.ALL_FLAGS)
+    out["all_flags_and_critpoints_ms"] = round(ms, 3)
+    log(f"relations: all flags + critical points {ms:.2f} ms")
+    return out
+
+
 def bench_irregular(t, torch, steps: int, seed: int = 5, check: bool = True) -> dict:
     """Kuhn 96^3 with 15 % of the tets stellar-split 1->4 at their centroid,
     plus 32 hub vertices each raised by 64 repeated splits around them, so
@@ -267,6 +298,8 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=60,
                     help="requests per arrival rate in the serving measurement (0: skip)")
     ap.add_argument("--rates", type=float, nargs="+", default=[100, 500, 1000, 2000, 5000])
+    ap.add_argument("--no-relations", action="store_true",
+                    help="skip the per-relation build times on the large mesh")
     ap.add_argument("--no-irregular", action="store_true",
                     help="skip the irregular-degree mesh (Kuhn 96^3, 15 %% split)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -374,6 +407,7 @@ def main() -> int:
     e2e_ms = (time.perf_counter() - t0) * 1e3
     d2.close()
 
+    relations = None if args.no_relations else bench_relations(t, torch, mesh)
     irregular = None if args.no_irregular else bench_irregular(t, torch, max(args.steps // 2, 1),
                                                                 check=not args.no_cpu)
     batch = bench_batch(t, torch, args.batch_meshes, 3) if args.batch_meshes > 0 else None
@@ -399,6 +433,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "end_to_end_ms": round(e2e_ms, 2),
         "end_to_end_note": "host cells/scalars in, VV CSR + classes out, incl. allocation (first use)",
+        "relations": relations,
         "irregular": irregular,
         "batched": batch,
         "serving": serve,

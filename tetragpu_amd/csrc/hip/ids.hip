@@ -398,32 +398,73 @@ __global__ void k_ef(const int32_t *__restrict__ ev, int64_t E, const int64_t *_
     }
 }
 
-// TT(t): union of VT rows of t's corners, minus t.  4-way merge of the
-// sorted rows with the four heads kept in registers: each step loads only
-// the heads it advances.
+// Merges of a few short sorted rows (TT, EE) stage the rows in LDS first:
+// one column of ROW_CAP words per thread (consecutive threads in
+// consecutive banks), filled by independent loads, so a row costs one
+// memory latency instead of one per element; the merge then reads LDS.
+// A thread whose rows exceed ROW_CAP merges from global memory.
+constexpr int ROW_THREADS = 64;
+constexpr int ROW_CAP = 128;
+
+__device__ __forceinline__ void stage_row(int32_t (*buf)[ROW_THREADS], int at,
+                                          const int32_t *__restrict__ src, int32_t n, int shift) {
+#pragma unroll 4
+    for (int32_t k = 0; k < n; k++) buf[at + k][threadIdx.x] = src[k] >> shift;
+}
+
+// TT(t): union of VT rows of t's corners, minus t (4-way merge).
 template <bool FILL>
-__global__ void k_tt(const int4 *__restrict__ tv4, int64_t T, const int32_t *__restrict__ vt_off,
-                     const int32_t *__restrict__ vtc, int64_t *__restrict__ cnt_or_off,
-                     int32_t *__restrict__ out) {
+__global__ __launch_bounds__(ROW_THREADS) void k_tt(const int4 *__restrict__ tv4, int64_t T,
+                                                    const int32_t *__restrict__ vt_off,
+                                                    const int32_t *__restrict__ vtc,
+                                                    int64_t *__restrict__ cnt_or_off,
+                                                    int32_t *__restrict__ out) {
+    __shared__ int32_t buf[ROW_CAP][ROW_THREADS];
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < T;
          t += (int64_t)gridDim.x * blockDim.x) {
         const int4 q = tv4[t];
         int32_t p0 = vt_off[q.x], e0 = vt_off[q.x + 1], p1 = vt_off[q.y], e1 = vt_off[q.y + 1];
         int32_t p2 = vt_off[q.z], e2 = vt_off[q.z + 1], p3 = vt_off[q.w], e3 = vt_off[q.w + 1];
-        int32_t h0 = p0 < e0 ? vtc[p0] >> 2 : INF, h1 = p1 < e1 ? vtc[p1] >> 2 : INF;
-        int32_t h2 = p2 < e2 ? vtc[p2] >> 2 : INF, h3 = p3 < e3 ? vtc[p3] >> 2 : INF;
         int64_t n = 0;
         int32_t *row = FILL ? out + cnt_or_off[t] : nullptr;
-        while (true) {
-            const int32_t mn = min(min(h0, h1), min(h2, h3));
-            if (mn == INF) break;
-            if (h0 == mn) h0 = ++p0 < e0 ? vtc[p0] >> 2 : INF;
-            if (h1 == mn) h1 = ++p1 < e1 ? vtc[p1] >> 2 : INF;
-            if (h2 == mn) h2 = ++p2 < e2 ? vtc[p2] >> 2 : INF;
-            if (h3 == mn) h3 = ++p3 < e3 ? vtc[p3] >> 2 : INF;
-            if (mn == (int32_t)t) continue;
-            if (FILL) row[n] = mn;
-            n++;
+        const int32_t n0 = e0 - p0, n1 = e1 - p1, n2 = e2 - p2, n3 = e3 - p3;
+        if (n0 + n1 + n2 + n3 <= ROW_CAP) {
+            stage_row(buf, 0, vtc + p0, n0, 2);
+            stage_row(buf, n0, vtc + p1, n1, 2);
+            stage_row(buf, n0 + n1, vtc + p2, n2, 2);
+            stage_row(buf, n0 + n1 + n2, vtc + p3, n3, 2);
+            // heads as LDS positions
+            int32_t a0 = 0, a1 = n0, a2 = n0 + n1, a3 = n0 + n1 + n2;
+            const int32_t z0 = n0, z1 = n0 + n1, z2 = n0 + n1 + n2, z3 = n0 + n1 + n2 + n3;
+            int32_t h0 = a0 < z0 ? buf[a0][threadIdx.x] : INF;
+            int32_t h1 = a1 < z1 ? buf[a1][threadIdx.x] : INF;
+            int32_t h2 = a2 < z2 ? buf[a2][threadIdx.x] : INF;
+            int32_t h3 = a3 < z3 ? buf[a3][threadIdx.x] : INF;
+            while (true) {
+                const int32_t mn = min(min(h0, h1), min(h2, h3));
+                if (mn == INF) break;
+                if (h0 == mn) h0 = ++a0 < z0 ? buf[a0][threadIdx.x] : INF;
+                if (h1 == mn) h1 = ++a1 < z1 ? buf[a1][threadIdx.x] : INF;
+                if (h2 == mn) h2 = ++a2 < z2 ? buf[a2][threadIdx.x] : INF;
+                if (h3 == mn) h3 = ++a3 < z3 ? buf[a3][threadIdx.x] : INF;
+                if (mn == (int32_t)t) continue;
+                if (FILL) row[n] = mn;
+                n++;
+            }
+        } else {
+            int32_t h0 = p0 < e0 ? vtc[p0] >> 2 : INF, h1 = p1 < e1 ? vtc[p1] >> 2 : INF;
+            int32_t h2 = p2 < e2 ? vtc[p2] >> 2 : INF, h3 = p3 < e3 ? vtc[p3] >> 2 : INF;
+            while (true) {
+                const int32_t mn = min(min(h0, h1), min(h2, h3));
+                if (mn == INF) break;
+                if (h0 == mn) h0 = ++p0 < e0 ? vtc[p0] >> 2 : INF;
+                if (h1 == mn) h1 = ++p1 < e1 ? vtc[p1] >> 2 : INF;
+                if (h2 == mn) h2 = ++p2 < e2 ? vtc[p2] >> 2 : INF;
+                if (h3 == mn) h3 = ++p3 < e3 ? vtc[p3] >> 2 : INF;
+                if (mn == (int32_t)t) continue;
+                if (FILL) row[n] = mn;
+                n++;
+            }
         }
         if (!FILL) cnt_or_off[t] = n;
     }
@@ -464,35 +505,52 @@ __global__ void k_ff(const int32_t *__restrict__ ft, int64_t F, const int32_t *_
     }
 }
 
-// EE(e): edges at lo or hi, minus e, ascending (rows of vv_eid, sorted here)
-template <bool FILL>
-__global__ void k_ee(const int32_t *__restrict__ ev, int64_t E, const int32_t *__restrict__ vv_off,
-                     const int32_t *__restrict__ vv_eid, int64_t *__restrict__ cnt_or_off,
-                     int32_t *__restrict__ out) {
+// EE(e): edges at lo or hi, minus e, ascending: merge of the two vertices'
+// edge-id lists (vv_eid rows sorted by id, in eids).  The lists share only
+// e itself, so the row length is deg(lo) + deg(hi) - 2.
+__global__ __launch_bounds__(ROW_THREADS) void k_ee(const int32_t *__restrict__ ev, int64_t E,
+                                                    const int32_t *__restrict__ vv_off,
+                                                    const int32_t *__restrict__ eids,
+                                                    const int64_t *__restrict__ off,
+                                                    int32_t *__restrict__ out) {
+    __shared__ int32_t buf[ROW_CAP][ROW_THREADS];
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
          e += (int64_t)gridDim.x * blockDim.x) {
         const int32_t a = ev[2 * e], b = ev[2 * e + 1];
-        const int64_t n = (vv_off[a + 1] - vv_off[a]) + (vv_off[b + 1] - vv_off[b]) - 2;
-        if (!FILL) {
-            cnt_or_off[e] = n;
-            continue;
-        }
-        int32_t *row = out + cnt_or_off[e];
+        int32_t i = vv_off[a], ie = vv_off[a + 1], j = vv_off[b], je = vv_off[b + 1];
+        int32_t *row = out + off[e];
         int64_t m = 0;
-        for (int side = 0; side < 2; side++) {
-            const int32_t v = side ? b : a;
-            for (int32_t i = vv_off[v]; i < vv_off[v + 1]; i++) {
-                const int32_t x = vv_eid[i];
-                if (x == (int32_t)e) continue;
-                int64_t k = m - 1;
-                while (k >= 0 && row[k] > x) {
-                    row[k + 1] = row[k];
-                    k--;
-                }
-                row[k + 1] = x;
-                m++;
+        const int32_t na = ie - i, nb = je - j;
+        if (na + nb <= ROW_CAP) {
+            stage_row(buf, 0, eids + i, na, 0);
+            stage_row(buf, na, eids + j, nb, 0);
+            i = 0, ie = na, j = na, je = na + nb;
+            int32_t x = i < ie ? buf[i][threadIdx.x] : INF, y = j < je ? buf[j][threadIdx.x] : INF;
+            while (x != INF || y != INF) {
+                const int32_t mn = min(x, y);
+                if (x == mn) x = ++i < ie ? buf[i][threadIdx.x] : INF;
+                if (y == mn) y = ++j < je ? buf[j][threadIdx.x] : INF;
+                if (mn != (int32_t)e) row[m++] = mn;
+            }
+        } else {
+            int32_t x = i < ie ? eids[i] : INF, y = j < je ? eids[j] : INF;
+            while (x != INF || y != INF) {
+                const int32_t mn = min(x, y);
+                if (x == mn) x = ++i < ie ? eids[i] : INF;
+                if (y == mn) y = ++j < je ? eids[j] : INF;
+                if (mn != (int32_t)e) row[m++] = mn;
             }
         }
+    }
+}
+
+// row lengths of EE: deg(lo) + deg(hi) - 2
+__global__ void k_ee_count(const int32_t *__restrict__ ev, int64_t E,
+                           const int32_t *__restrict__ vv_off, int64_t *__restrict__ cnt) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t a = ev[2 * e], b = ev[2 * e + 1];
+        cnt[e] = (int64_t)(vv_off[a + 1] - vv_off[a]) + (vv_off[b + 1] - vv_off[b]) - 2;
     }
 }
 
@@ -788,7 +846,7 @@ int ensure_tt(tg_mesh &m) {
     if ((rc = m.tt_off.ensure(sizeof(int64_t) * (size_t)(T + 1)))) return rc;
     int64_t *off = m.tt_off.as<int64_t>();
     if (T > 0) {
-        hipLaunchKernelGGL((k_tt<false>), dim3(grid(T)), dim3(256), 0, m.stream,
+        hipLaunchKernelGGL((k_tt<false>), dim3(grid(T, ROW_THREADS)), dim3(ROW_THREADS), 0, m.stream,
                            m.tv.as<const int4>(), T, m.vt_off.as<int32_t>(), m.vtc.as<int32_t>(),
                            off, nullptr);
         TG_LAUNCH_CHECK("k_tt count");
@@ -797,7 +855,7 @@ int ensure_tt(tg_mesh &m) {
     if ((rc = scan_counts64(m, off, T, nnz))) return rc;
     if ((rc = m.tt_vals.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)))) return rc;
     if (T > 0) {
-        hipLaunchKernelGGL((k_tt<true>), dim3(grid(T)), dim3(256), 0, m.stream,
+        hipLaunchKernelGGL((k_tt<true>), dim3(grid(T, ROW_THREADS)), dim3(ROW_THREADS), 0, m.stream,
                            m.tv.as<const int4>(), T, m.vt_off.as<int32_t>(), m.vtc.as<int32_t>(),
                            off, m.tt_vals.as<int32_t>());
         TG_LAUNCH_CHECK("k_tt fill");
@@ -836,23 +894,30 @@ int ensure_ee(tg_mesh &m) {
     if (m.ee_built) return OK;
     int rc;
     if ((rc = ensure_edges(m))) return rc;
-    const int64_t E = m.n_edges;
+    const int64_t E = m.n_edges, V = m.n_points;
     if ((rc = m.ee_off.ensure(sizeof(int64_t) * (size_t)(E + 1)))) return rc;
     int64_t *off = m.ee_off.as<int64_t>();
     if (E > 0) {
-        hipLaunchKernelGGL((k_ee<false>), dim3(grid(E)), dim3(256), 0, m.stream,
-                           m.ev.as<int32_t>(), E, m.vv_off.as<int32_t>(), m.vv_eid.as<int32_t>(),
-                           off, nullptr);
-        TG_LAUNCH_CHECK("k_ee count");
+        hipLaunchKernelGGL(k_ee_count, dim3(grid(E)), dim3(256), 0, m.stream, m.ev.as<int32_t>(),
+                           E, m.vv_off.as<int32_t>(), off);
+        TG_LAUNCH_CHECK("k_ee_count");
     }
     int64_t nnz = 0;
     if ((rc = scan_counts64(m, off, E, nnz))) return rc;
     if ((rc = m.ee_vals.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)))) return rc;
     if (E > 0) {
-        hipLaunchKernelGGL((k_ee<true>), dim3(grid(E)), dim3(256), 0, m.stream,
-                           m.ev.as<int32_t>(), E, m.vv_off.as<int32_t>(), m.vv_eid.as<int32_t>(),
+        // each vertex's edge ids (the vv_eid row) sorted ascending
+        const int64_t n2 = 2 * E;
+        if ((rc = m.tmp_c.ensure(sizeof(int32_t) * (size_t)n2))) return rc;
+        TG_HIP(hipMemcpyAsync(m.tmp_c.p, m.vv_eid.p, sizeof(int32_t) * (size_t)n2,
+                              hipMemcpyDeviceToDevice, m.stream));
+        if ((rc = launch_sort_rows_i32(m.vv_off.as<int32_t>(), m.tmp_c.as<int32_t>(), V,
+                                       m.stream)))
+            return rc;
+        hipLaunchKernelGGL(k_ee, dim3(grid(E, ROW_THREADS)), dim3(ROW_THREADS), 0, m.stream,
+                           m.ev.as<int32_t>(), E, m.vv_off.as<int32_t>(), m.tmp_c.as<int32_t>(),
                            off, m.ee_vals.as<int32_t>());
-        TG_LAUNCH_CHECK("k_ee fill");
+        TG_LAUNCH_CHECK("k_ee");
     }
     m.ee_nnz = nnz;
     m.ee_built = true;
