@@ -45,13 +45,24 @@ __device__ __forceinline__ void face_pos(int s, int &lo, int &mid, int &hi) {
     hi = s == 0 ? 2 : 3;
 }
 
-// index of x in the ascending array a[0..n), or -1
+// index of x in the ascending array a[0..n), or -1.  Rows shorter than 128
+// take seven fixed steps (no data-dependent loop), so several searches of
+// one thread overlap their loads; longer rows bisect.
 __device__ __forceinline__ int32_t bsearch_i32(const int32_t *a, int32_t n, int32_t x) {
-    int32_t lo = 0, hi = n;
-    while (lo < hi) {
-        const int32_t mid = (lo + hi) >> 1;
-        if (a[mid] < x) lo = mid + 1;
-        else hi = mid;
+    int32_t lo = 0;
+    if (n < 128) {
+#pragma unroll
+        for (int step = 64; step >= 1; step >>= 1) {
+            const int32_t probe = lo + step - 1;
+            if (probe < n && a[probe] < x) lo += step;
+        }
+    } else {
+        int32_t hi = n;
+        while (lo < hi) {
+            const int32_t mid = (lo + hi) >> 1;
+            if (a[mid] < x) lo = mid + 1;
+            else hi = mid;
+        }
     }
     return (lo < n && a[lo] == x) ? lo : -1;
 }
@@ -193,10 +204,16 @@ __global__ void k_te(const int4 *__restrict__ tv4, int64_t T, const int32_t *__r
 }
 
 // ---- ET = VT(lo) n VT(hi) -------------------------------------------------------
+// Both rows are staged in LDS when they fit (see stage_row), then merged.
+constexpr int ET_THREADS = 64, ET_CAP = 128;
+
 template <bool FILL>
-__global__ void k_et(const int32_t *__restrict__ ev, int64_t E, int64_t V,
-                     const int32_t *__restrict__ vt_off, const int32_t *__restrict__ vtc,
-                     int64_t *__restrict__ cnt_or_off, int32_t *__restrict__ out) {
+__global__ __launch_bounds__(ET_THREADS) void k_et(const int32_t *__restrict__ ev, int64_t E,
+                                                   int64_t V, const int32_t *__restrict__ vt_off,
+                                                   const int32_t *__restrict__ vtc,
+                                                   int64_t *__restrict__ cnt_or_off,
+                                                   int32_t *__restrict__ out) {
+    __shared__ int32_t buf[ET_CAP][ET_THREADS];
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
          e += (int64_t)gridDim.x * blockDim.x) {
         const int32_t a = ev[2 * e], b = ev[2 * e + 1];
@@ -204,15 +221,34 @@ __global__ void k_et(const int32_t *__restrict__ ev, int64_t E, int64_t V,
         int32_t i = ok ? vt_off[a] : 0, ie = ok ? vt_off[a + 1] : 0;
         int32_t j = ok ? vt_off[b] : 0, je = ok ? vt_off[b + 1] : 0;
         int64_t n = 0, o = FILL ? cnt_or_off[e] : 0;
-        int32_t x = i < ie ? vtc[i] >> 2 : INF, y = j < je ? vtc[j] >> 2 : INF;
-        while (x != INF && y != INF) {
-            if (x == y) {
-                if (FILL) out[o + n] = x;
-                n++;
+        const int32_t na = ie - i, nb = je - j;
+        if (na + nb <= ET_CAP) {
+#pragma unroll 4
+            for (int32_t k = 0; k < na; k++) buf[k][threadIdx.x] = vtc[i + k] >> 2;
+#pragma unroll 4
+            for (int32_t k = 0; k < nb; k++) buf[na + k][threadIdx.x] = vtc[j + k] >> 2;
+            i = 0, ie = na, j = na, je = na + nb;
+            int32_t x = i < ie ? buf[i][threadIdx.x] : INF, y = j < je ? buf[j][threadIdx.x] : INF;
+            while (x != INF && y != INF) {
+                if (x == y) {
+                    if (FILL) out[o + n] = x;
+                    n++;
+                }
+                const int32_t m = min(x, y);
+                if (x == m) x = ++i < ie ? buf[i][threadIdx.x] : INF;
+                if (y == m) y = ++j < je ? buf[j][threadIdx.x] : INF;
             }
-            const int32_t m = min(x, y);
-            if (x == m) x = ++i < ie ? vtc[i] >> 2 : INF;
-            if (y == m) y = ++j < je ? vtc[j] >> 2 : INF;
+        } else {
+            int32_t x = i < ie ? vtc[i] >> 2 : INF, y = j < je ? vtc[j] >> 2 : INF;
+            while (x != INF && y != INF) {
+                if (x == y) {
+                    if (FILL) out[o + n] = x;
+                    n++;
+                }
+                const int32_t m = min(x, y);
+                if (x == m) x = ++i < ie ? vtc[i] >> 2 : INF;
+                if (y == m) y = ++j < je ? vtc[j] >> 2 : INF;
+            }
         }
         if (!FILL) cnt_or_off[e] = n;
     }
@@ -369,17 +405,29 @@ __device__ __forceinline__ void insert_sorted(int32_t *row, int64_t &m, int32_t 
     m++;
 }
 
-// EF(e): faces holding e, from the two faces of e in each cell of ET(e)
+// EF(e): faces holding e, from the two faces of e in each cell of ET(e).
+// The sorted, duplicate-free row is built in an LDS column when 2|ET(e)|
+// fits (else in global memory: tmp for the count pass, the row itself for
+// the fill pass).
+constexpr int EF_CAP = 64;
+
 template <bool FILL>
-__global__ void k_ef(const int32_t *__restrict__ ev, int64_t E, const int64_t *__restrict__ et_off,
-                     const int32_t *__restrict__ et, const int32_t *__restrict__ te,
-                     const int32_t *__restrict__ tf, int64_t *__restrict__ cnt_or_off,
-                     int32_t *__restrict__ out, int32_t *__restrict__ tmp) {
+__global__ __launch_bounds__(ET_THREADS) void k_ef(const int32_t *__restrict__ ev, int64_t E,
+                                                   const int64_t *__restrict__ et_off,
+                                                   const int32_t *__restrict__ et,
+                                                   const int32_t *__restrict__ te,
+                                                   const int32_t *__restrict__ tf,
+                                                   int64_t *__restrict__ cnt_or_off,
+                                                   int32_t *__restrict__ out,
+                                                   int32_t *__restrict__ tmp) {
+    __shared__ int32_t col[EF_CAP][ET_THREADS];
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
          e += (int64_t)gridDim.x * blockDim.x) {
-        int32_t *row = FILL ? out + cnt_or_off[e] : tmp + 2 * et_off[e];
+        const int64_t b0 = et_off[e], b1 = et_off[e + 1];
+        const bool lds = 2 * (b1 - b0) <= EF_CAP;
+        int32_t *grow = FILL ? out + cnt_or_off[e] : tmp + 2 * b0;
         int64_t m = 0;
-        for (int64_t i = et_off[e]; i < et_off[e + 1]; i++) {
+        for (int64_t i = b0; i < b1; i++) {
             const int32_t t = et[i];
             int k = 0;
             for (; k < 6; k++)
@@ -391,9 +439,22 @@ __global__ void k_ef(const int32_t *__restrict__ ev, int64_t E, const int64_t *_
                 face_pos(s, lo, mid, hi);
                 const bool has_i = lo == pi[k] || mid == pi[k] || hi == pi[k];
                 const bool has_j = lo == pj[k] || mid == pj[k] || hi == pj[k];
-                if (has_i && has_j) insert_sorted(row, m, tf[4 * (int64_t)t + s]);
+                if (!(has_i && has_j)) continue;
+                const int32_t x = tf[4 * (int64_t)t + s];
+                if (lds) {  // insert into the ascending column unless present
+                    int64_t p = m;
+                    while (p > 0 && col[p - 1][threadIdx.x] > x) p--;
+                    if (p > 0 && col[p - 1][threadIdx.x] == x) continue;
+                    for (int64_t q = m; q > p; q--) col[q][threadIdx.x] = col[q - 1][threadIdx.x];
+                    col[p][threadIdx.x] = x;
+                    m++;
+                } else {
+                    insert_sorted(grow, m, x);
+                }
             }
         }
+        if (FILL && lds)
+            for (int64_t q = 0; q < m; q++) grow[q] = col[q][threadIdx.x];
         if (!FILL) cnt_or_off[e] = m;
     }
 }
@@ -704,7 +765,7 @@ int ensure_et(tg_mesh &m) {
     if ((rc = m.et_off.ensure(sizeof(int64_t) * (size_t)(E + 1)))) return rc;
     int64_t *off = m.et_off.as<int64_t>();
     if (E > 0) {
-        hipLaunchKernelGGL((k_et<false>), dim3(grid(E)), dim3(256), 0, m.stream,
+        hipLaunchKernelGGL((k_et<false>), dim3(grid(E, ET_THREADS)), dim3(ET_THREADS), 0, m.stream,
                            m.ev.as<int32_t>(), E, m.n_points, m.vt_off.as<int32_t>(),
                            m.vtc.as<int32_t>(), off, nullptr);
         TG_LAUNCH_CHECK("k_et count");
@@ -713,7 +774,7 @@ int ensure_et(tg_mesh &m) {
     if ((rc = scan_counts64(m, off, E, nnz))) return rc;
     if ((rc = m.et_vals.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)))) return rc;
     if (E > 0) {
-        hipLaunchKernelGGL((k_et<true>), dim3(grid(E)), dim3(256), 0, m.stream,
+        hipLaunchKernelGGL((k_et<true>), dim3(grid(E, ET_THREADS)), dim3(ET_THREADS), 0, m.stream,
                            m.ev.as<int32_t>(), E, m.n_points, m.vt_off.as<int32_t>(),
                            m.vtc.as<int32_t>(), off, m.et_vals.as<int32_t>());
         TG_LAUNCH_CHECK("k_et fill");
@@ -817,7 +878,7 @@ int ensure_ef(tg_mesh &m) {
         return rc;
     int64_t *off = m.ef_off.as<int64_t>();
     if (E > 0) {
-        hipLaunchKernelGGL((k_ef<false>), dim3(grid(E)), dim3(256), 0, m.stream,
+        hipLaunchKernelGGL((k_ef<false>), dim3(grid(E, ET_THREADS)), dim3(ET_THREADS), 0, m.stream,
                            m.ev.as<int32_t>(), E, m.et_off.as<int64_t>(), m.et_vals.as<int32_t>(),
                            m.te.as<int32_t>(), m.tf.as<int32_t>(), off, nullptr,
                            m.tmp_c.as<int32_t>());
@@ -827,7 +888,7 @@ int ensure_ef(tg_mesh &m) {
     if ((rc = scan_counts64(m, off, E, nnz))) return rc;
     if ((rc = m.ef_vals.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)))) return rc;
     if (E > 0) {
-        hipLaunchKernelGGL((k_ef<true>), dim3(grid(E)), dim3(256), 0, m.stream,
+        hipLaunchKernelGGL((k_ef<true>), dim3(grid(E, ET_THREADS)), dim3(ET_THREADS), 0, m.stream,
                            m.ev.as<int32_t>(), E, m.et_off.as<int64_t>(), m.et_vals.as<int32_t>(),
                            m.te.as<int32_t>(), m.tf.as<int32_t>(), off, m.ef_vals.as<int32_t>(),
                            nullptr);
