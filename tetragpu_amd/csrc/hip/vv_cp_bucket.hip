@@ -117,18 +117,22 @@ __global__ __launch_bounds__(L1_THREADS) void k_l1_count(const int4 *__restrict_
 
 // Staged scatter shared by both routing levels: a tile of records is
 // counting-sorted by destination bin in LDS, then each bin's run is written
-// with consecutive lanes to consecutive addresses.
-constexpr int STAGE = 4096;  // records per tile
+// with consecutive lanes to consecutive addresses.  A record's bin is
+// recomputed from its vertex field, so a tile costs 12 bytes of LDS per
+// record; larger tiles give longer runs per bin and fewer partial lines.
+constexpr int STAGE1 = 8192;   // level-1 records per tile (2048 tets)
+constexpr int STAGE2 = 12288;  // level-2 records per tile
 
+template <int N>
 struct Stage {
-    uint32_t w0[STAGE], w1[STAGE], w2[STAGE];
-    uint16_t bin[STAGE];
+    uint32_t w0[N], w1[N], w2[N];
     int32_t cnt[MAX_FINE], toff[MAX_FINE], gbase[MAX_FINE];
     int32_t total;
 };
 
 // Exclusive scan of cnt[0..nbin) into toff, total into st.total (block-wide).
-__device__ __forceinline__ void stage_scan(Stage &st, int nbin) {
+template <typename ST>
+__device__ __forceinline__ void stage_scan(ST &st, int nbin) {
     if (threadIdx.x < 64) {
         const int lane = threadIdx.x;
         int32_t carry = 0;
@@ -144,13 +148,16 @@ __device__ __forceinline__ void stage_scan(Stage &st, int nbin) {
 }
 
 // Write the staged tile: consecutive slots of one bin go to consecutive
-// addresses of that bin's range.
-__device__ __forceinline__ void stage_flush(Stage &st, Rec out, int nthreads) {
+// addresses of that bin's range; the bin of a slot is (v >> shift) - base.
+template <typename ST>
+__device__ __forceinline__ void stage_flush(ST &st, Rec out, int nthreads, int shift,
+                                            int32_t base) {
     const int32_t total = st.total;
     for (int slot = threadIdx.x; slot < total; slot += nthreads) {
-        const int b = st.bin[slot];
+        const uint32_t w0 = st.w0[slot];
+        const int b = (int)((w0 & M24) >> shift) - base;
         const int64_t pos = (int64_t)st.gbase[b] + (slot - st.toff[b]);
-        out.w0[pos] = st.w0[slot];
+        out.w0[pos] = w0;
         out.w1[pos] = st.w1[slot];
         out.w2[pos] = st.w2[slot];
     }
@@ -161,46 +168,61 @@ __global__ __launch_bounds__(L1_THREADS) void k_l1_scatter(const int4 *__restric
                                                             int32_t nch,
                                                             const int32_t *__restrict__ h1s,
                                                             Rec out) {
-    __shared__ Stage st;
+    constexpr int TILE = STAGE1 / 4;          // tets per tile
+    constexpr int TPT = TILE / L1_THREADS;    // tets per thread
+    __shared__ Stage<STAGE1> st;
     const int tid = threadIdx.x;
     for (int i = tid; i < nc; i += L1_THREADS) st.gbase[i] = h1s[(int64_t)i * nch + blockIdx.x];
     const int64_t t0 = (int64_t)blockIdx.x * tpc, t1 = min(T, t0 + tpc);
-    int4 qn = make_int4(0, 0, 0, 0);
-    if (tid < STAGE / 4 && t0 + tid < t1) qn = tv4[t0 + tid];
-    for (int64_t tb = t0; tb < t1; tb += STAGE / 4) {
+    int4 qn[TPT];
+#pragma unroll
+    for (int k = 0; k < TPT; k++) {
+        const int64_t t = t0 + k * L1_THREADS + tid;
+        qn[k] = t < t1 ? tv4[t] : make_int4(0, 0, 0, 0);
+    }
+    for (int64_t tb = t0; tb < t1; tb += TILE) {
         for (int i = tid; i < nc; i += L1_THREADS) st.cnt[i] = 0;
         __syncthreads();
-        const int64_t t = tb + tid;
-        const bool on = tid < STAGE / 4 && t < t1;
-        const int4 q = qn;
-        int r0 = 0, r1 = 0, r2 = 0, r3 = 0;
-        if (on) {
-            r0 = atomicAdd(&st.cnt[q.x >> cshift], 1);
-            r1 = atomicAdd(&st.cnt[q.y >> cshift], 1);
-            r2 = atomicAdd(&st.cnt[q.z >> cshift], 1);
-            r3 = atomicAdd(&st.cnt[q.w >> cshift], 1);
+        int4 q[TPT];
+        bool on[TPT];
+        int r[TPT][4];
+#pragma unroll
+        for (int k = 0; k < TPT; k++) {
+            q[k] = qn[k];
+            on[k] = tb + k * L1_THREADS + tid < t1;
+            if (on[k]) {
+                r[k][0] = atomicAdd(&st.cnt[q[k].x >> cshift], 1);
+                r[k][1] = atomicAdd(&st.cnt[q[k].y >> cshift], 1);
+                r[k][2] = atomicAdd(&st.cnt[q[k].z >> cshift], 1);
+                r[k][3] = atomicAdd(&st.cnt[q[k].w >> cshift], 1);
+            }
         }
         __syncthreads();
         stage_scan(st, nc);
         __syncthreads();
-        if (on) {
-            const uint32_t x = q.x, y = q.y, z = q.z, w = q.w;
+#pragma unroll
+        for (int k = 0; k < TPT; k++) {
+            if (!on[k]) continue;
+            const uint32_t x = q[k].x, y = q[k].y, z = q[k].z, w = q[k].w;
             auto put = [&](int bin, int rank, uint32_t v, uint32_t a, uint32_t b, uint32_t c) {
                 const int slot = st.toff[bin] + rank;
                 st.w0[slot] = v | (a << 24);
                 st.w1[slot] = (a >> 8) | (b << 16);
                 st.w2[slot] = (b >> 16) | (c << 8);
-                st.bin[slot] = (uint16_t)bin;
             };
-            put(q.x >> cshift, r0, x, y, z, w);
-            put(q.y >> cshift, r1, y, x, z, w);
-            put(q.z >> cshift, r2, z, x, y, w);
-            put(q.w >> cshift, r3, w, x, y, z);
+            put(q[k].x >> cshift, r[k][0], x, y, z, w);
+            put(q[k].y >> cshift, r[k][1], y, x, z, w);
+            put(q[k].z >> cshift, r[k][2], z, x, y, w);
+            put(q[k].w >> cshift, r[k][3], w, x, y, z);
         }
         __syncthreads();
         // next tile's tets are in flight during the flush
-        if (tid < STAGE / 4 && tb + STAGE / 4 + tid < t1) qn = tv4[tb + STAGE / 4 + tid];
-        stage_flush(st, out, L1_THREADS);
+#pragma unroll
+        for (int k = 0; k < TPT; k++) {
+            const int64_t t = tb + TILE + k * L1_THREADS + tid;
+            if (t < t1) qn[k] = tv4[t];
+        }
+        stage_flush(st, out, L1_THREADS, cshift, 0);
         __syncthreads();
         for (int i = tid; i < nc; i += L1_THREADS) st.gbase[i] += st.cnt[i];
     }
@@ -257,7 +279,7 @@ __global__ __launch_bounds__(L2_THREADS) void k_l2_scatter(const Rec in,
                                                             int32_t nb,
                                                             const int32_t *__restrict__ h2s,
                                                             Rec out) {
-    __shared__ Stage st;
+    __shared__ Stage<STAGE2> st;
     const int tid = threadIdx.x;
     const int32_t c = blockIdx.x, s = blockIdx.y, S = gridDim.y;
     const int32_t fb0 = c << fs;
@@ -265,7 +287,7 @@ __global__ __launch_bounds__(L2_THREADS) void k_l2_scatter(const Rec in,
     for (int i = tid; i < F; i += L2_THREADS) st.gbase[i] = h2s[(int64_t)(fb0 + i) * S + s];
     int32_t lo, hi;
     slice_range(h1s, nch, c, s, S, lo, hi);
-    constexpr int PER = STAGE / L2_THREADS;
+    constexpr int PER = STAGE2 / L2_THREADS;
     uint32_t a[PER], b[PER], d[PER];
     auto load = [&](int32_t ib) {
 #pragma unroll
@@ -280,7 +302,7 @@ __global__ __launch_bounds__(L2_THREADS) void k_l2_scatter(const Rec in,
         }
     };
     load(lo);
-    for (int32_t ib = lo; ib < hi; ib += STAGE) {
+    for (int32_t ib = lo; ib < hi; ib += STAGE2) {
         for (int i = tid; i < F; i += L2_THREADS) st.cnt[i] = 0;
         __syncthreads();
         int f[PER], r[PER];
@@ -299,11 +321,10 @@ __global__ __launch_bounds__(L2_THREADS) void k_l2_scatter(const Rec in,
                 st.w0[slot] = a[k];
                 st.w1[slot] = b[k];
                 st.w2[slot] = d[k];
-                st.bin[slot] = (uint16_t)f[k];
             }
         __syncthreads();
-        load(ib + STAGE);  // next tile's records are in flight during the flush
-        stage_flush(st, out, L2_THREADS);
+        load(ib + STAGE2);  // next tile's records are in flight during the flush
+        stage_flush(st, out, L2_THREADS, bv_log, fb0);
         __syncthreads();
         for (int i = tid; i < F; i += L2_THREADS) st.gbase[i] += st.cnt[i];
     }
