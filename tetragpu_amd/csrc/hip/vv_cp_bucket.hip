@@ -1382,7 +1382,8 @@ BucketPlan plan_buckets(int64_t n_cells, int64_t n_points, int bv_log, int64_t t
     p.ok = V > 0 && V <= (int64_t)M24 + 1 && (1 << p.fs) <= MAX_FINE && p.nc <= MAX_COARSE;
     p.tpc = std::max<int64_t>(1024, tpc);
     p.nchunks = (int32_t)std::max<int64_t>(1, (T + p.tpc - 1) / p.tpc);
-    p.slices = std::max(1, std::min(64, (1024 + p.nc - 1) / p.nc));
+    // about 2048 level-2 workgroups: several rounds of one resident block per CU
+    p.slices = std::max(1, std::min(64, (2048 + p.nc - 1) / p.nc));
     return p;
 }
 
