@@ -507,19 +507,29 @@ __global__ __launch_bounds__(64) void k_cp_fix_desc(
     const int lane = threadIdx.x & 31, half = threadIdx.x >> 5;
     const int32_t n = counters[tg::C_FIXD];
     const uint64_t hmask = half ? 0xFFFFFFFF00000000ull : 0xFFFFFFFFull;
+    // the next vertex's id and row bounds are loaded during the current one
+    // (the pass is bound by chains of dependent loads)
+    auto head = [&](int32_t i, int32_t &v, int32_t &b, int32_t &e, int32_t &rs) {
+        v = i < n ? fixd[i] : 0;
+        b = i < n ? off[v] : 0;
+        e = i < n ? off[v + 1] : 0;
+        rs = i < n ? rowsrc[v] : 0;
+    };
+    int32_t nv_, nb_, ne_, nrs_;
+    head(2 * blockIdx.x + half, nv_, nb_, ne_, nrs_);
     for (int32_t i0 = 2 * blockIdx.x; i0 < n; i0 += 2 * gridDim.x) {
         const int32_t i = i0 + half;
         const bool live = i < n;
-        const int32_t v = live ? fixd[i] : 0;
-        const int32_t b = live ? off[v] : 0;
-        int d = live ? off[v + 1] - b : 0;
+        const int32_t v = nv_, b = nb_, rs = nrs_;
+        int d = live ? ne_ - b : 0;
+        head(i + 2 * gridDim.x, nv_, nb_, ne_, nrs_);
         if (d > 32) {  // not from the bucketed kernel: generic pass
             if (lane == 0) fix[atomicAdd(counters + C_FIX, 1)] = v;
             d = 0;
         }
         const bool on = lane < d;
         const int32_t a = on ? cols[b + lane] : INT32_MAX;
-        const uint32_t ds = on ? rowdesc[rowsrc[v] + lane] : 0u;
+        const uint32_t ds = on ? rowdesc[rs + lane] : 0u;
         const int cls = (int)(ds >> 5), comp = (int)(ds & 31u);
         nbs[half][lane] = a;
         dsc[half][lane] = (uint8_t)ds;
