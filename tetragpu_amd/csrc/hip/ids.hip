@@ -69,10 +69,14 @@ __device__ __forceinline__ int32_t bsearch_i32(const int32_t *a, int32_t n, int3
 
 // ---- edges --------------------------------------------------------------------
 // Lane per vertex v: first position of every upper edge (v, u>v), flag it.
+// Every pair of v's cells whose low vertex is v also records the VV index of
+// its edge in te (replaced by the edge id once ids exist): the row search is
+// done here, on v's own row, instead of once per cell on random rows.
 __global__ void k_edge_first(const int4 *__restrict__ tv4, const int32_t *__restrict__ vt_off,
                              const int32_t *__restrict__ vtc, const int32_t *__restrict__ vv_off,
                              const int32_t *__restrict__ vv, int64_t V,
-                             int32_t *__restrict__ efirst, uint8_t *__restrict__ flags) {
+                             int32_t *__restrict__ efirst, uint8_t *__restrict__ flags,
+                             int32_t *__restrict__ te_idx) {
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
          v += (int64_t)gridDim.x * blockDim.x) {
         const int32_t rb = vv_off[v], rn = vv_off[v + 1] - rb;
@@ -87,8 +91,9 @@ __global__ void k_edge_first(const int4 *__restrict__ tv4, const int32_t *__rest
                 const int32_t u = corner(q, j);
                 if (u < v) continue;
                 const int32_t idx = bsearch_i32(row, rn, u);
-                if (idx >= 0 && ef[idx] == INF)
-                    ef[idx] = 6 * t + pair_index(min(j, jv), max(j, jv));
+                const int32_t pos = 6 * t + pair_index(min(j, jv), max(j, jv));
+                te_idx[pos] = idx >= 0 ? rb + idx : -1;
+                if (idx >= 0 && ef[idx] == INF) ef[idx] = pos;
             }
         }
         for (int32_t i = 0; i < rn; i++)
@@ -190,16 +195,12 @@ __device__ __forceinline__ int32_t edge_id(const int32_t *vv_off, const int32_t 
     return j >= 0 ? vv_eid[rb + j] : -1;
 }
 
-__global__ void k_te(const int4 *__restrict__ tv4, int64_t T, const int32_t *__restrict__ vv_off,
-                     const int32_t *__restrict__ vv, const int32_t *__restrict__ vv_eid,
-                     int32_t *__restrict__ te) {
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < T;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        const int4 q = tv4[t];
-        int k = 0;
-        for (int i = 0; i < 3; i++)
-            for (int j = i + 1; j < 4; j++, k++)
-                te[6 * t + k] = edge_id(vv_off, vv, vv_eid, corner(q, i), corner(q, j));
+// TE: the VV indices recorded by k_edge_first become edge ids
+__global__ void k_te(int32_t *__restrict__ te, int64_t n, const int32_t *__restrict__ vv_eid) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t k = te[i];
+        te[i] = k >= 0 ? vv_eid[k] : -1;
     }
 }
 
@@ -699,6 +700,7 @@ int ensure_edges(tg_mesh &m) {
     const size_t I = sizeof(int32_t);
     if ((rc = m.edge_first.ensure(I * (size_t)std::max(nnz, 1))) ||
         (rc = m.vv_eid.ensure(I * (size_t)std::max(nnz, 1))) ||
+        (rc = m.te.ensure(I * 6 * (size_t)std::max<int64_t>(T, 1))) ||
         (rc = m.tmp_a.ensure(std::max<size_t>(6 * (size_t)T, 1))) ||  // flags
         (rc = m.ve_off.ensure(I * (size_t)(V + 1))))
         return rc;
@@ -711,7 +713,7 @@ int ensure_edges(tg_mesh &m) {
         hipLaunchKernelGGL(k_edge_first, dim3(grid(V)), dim3(256), 0, m.stream,
                            m.tv.as<const int4>(), m.vt_off.as<int32_t>(), m.vtc.as<int32_t>(),
                            m.vv_off.as<int32_t>(), m.vv_cols.as<int32_t>(), V,
-                           m.edge_first.as<int32_t>(), flags);
+                           m.edge_first.as<int32_t>(), flags, m.te.as<int32_t>());
         TG_LAUNCH_CHECK("k_edge_first");
     }
     int32_t *bits, *wpre;
@@ -719,8 +721,7 @@ int ensure_edges(tg_mesh &m) {
     if ((rc = build_ranks(m, flags, 6 * T, bits, wpre, E))) return rc;
     m.n_edges = E;
     if ((rc = m.ev.ensure(I * 2 * (size_t)std::max<int64_t>(E, 1))) ||
-        (rc = m.ve_vals.ensure(I * 2 * (size_t)std::max<int64_t>(E, 1))) ||
-        (rc = m.te.ensure(I * 6 * (size_t)std::max<int64_t>(T, 1))))
+        (rc = m.ve_vals.ensure(I * 2 * (size_t)std::max<int64_t>(E, 1))))
         return rc;
     const int4 *tv4 = m.tv.as<const int4>();
     int32_t *voff = m.ve_off.as<int32_t>();
@@ -747,9 +748,8 @@ int ensure_edges(tg_mesh &m) {
         TG_LAUNCH_CHECK("k_edge_lower");
     }
     if (T > 0) {
-        hipLaunchKernelGGL(k_te, dim3(grid(T)), dim3(256), 0, m.stream, m.tv.as<const int4>(), T,
-                           m.vv_off.as<int32_t>(), m.vv_cols.as<int32_t>(),
-                           m.vv_eid.as<int32_t>(), m.te.as<int32_t>());
+        hipLaunchKernelGGL(k_te, dim3(grid(6 * T)), dim3(256), 0, m.stream, m.te.as<int32_t>(),
+                           6 * T, m.vv_eid.as<int32_t>());
         TG_LAUNCH_CHECK("k_te");
     }
     m.edges_built = true;
