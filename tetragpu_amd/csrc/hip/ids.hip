@@ -187,14 +187,6 @@ __global__ void k_edge_lower(const int32_t *__restrict__ vv_off, const int32_t *
     }
 }
 
-__device__ __forceinline__ int32_t edge_id(const int32_t *vv_off, const int32_t *vv,
-                                           const int32_t *vv_eid, int32_t a, int32_t b) {
-    const int32_t lo = min(a, b), hi = max(a, b);
-    const int32_t rb = vv_off[lo];
-    const int32_t j = bsearch_i32(vv + rb, vv_off[lo + 1] - rb, hi);
-    return j >= 0 ? vv_eid[rb + j] : -1;
-}
-
 // TE: the VV indices recorded by k_edge_first become edge ids
 __global__ void k_te(int32_t *__restrict__ te, int64_t n, const int32_t *__restrict__ vv_eid) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -381,18 +373,39 @@ __global__ void k_vf(const int4 *__restrict__ tv4, const int32_t *__restrict__ v
     }
 }
 
-// FE[f] = {max vertex, edge id} for the edges (lo,mid), (lo,hi), (mid,hi)
-__global__ void k_fe(const int32_t *__restrict__ fv, int64_t F, const int32_t *__restrict__ vv_off,
-                     const int32_t *__restrict__ vv, const int32_t *__restrict__ vv_eid,
+// FE[f] = {max vertex, edge id} for the edges (lo,mid), (lo,hi), (mid,hi),
+// read from TE of the face's first cell (FT[f][0]) at the slot whose
+// positional triple is the face's.
+__global__ void k_fe(const int4 *__restrict__ tv4, const int32_t *__restrict__ fv, int64_t F,
+                     const int32_t *__restrict__ ft, const int32_t *__restrict__ te,
                      int32_t *__restrict__ fe) {
     for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < F;
          f += (int64_t)gridDim.x * blockDim.x) {
         const int32_t mid = fv[3 * f], hi = fv[3 * f + 1], lo = fv[3 * f + 2];
-        const int32_t pa[3] = {lo, lo, mid}, pb[3] = {mid, hi, hi};
-        for (int e = 0; e < 3; e++) {
-            fe[6 * f + 2 * e] = max(pa[e], pb[e]);
-            fe[6 * f + 2 * e + 1] = edge_id(vv_off, vv, vv_eid, pa[e], pb[e]);
+        const int32_t t = ft[2 * f];
+        if (t < 0) {  // not reached for a built FT: every face has a first cell
+            for (int k = 0; k < 6; k++) fe[6 * f + k] = -1;
+            continue;
         }
+        const int4 q = tv4[t];
+        int s = 0;
+        for (int k = 0; k < 4; k++) {
+            int l, m, h;
+            face_pos(k, l, m, h);
+            if (corner(q, l) == lo && corner(q, m) == mid && corner(q, h) == hi) {
+                s = k;
+                break;
+            }
+        }
+        int pl, pm, ph;
+        face_pos(s, pl, pm, ph);
+        const int32_t *tr = te + 6 * (int64_t)t;
+        fe[6 * f] = max(lo, mid);
+        fe[6 * f + 1] = tr[pair_index(pl, pm)];
+        fe[6 * f + 2] = max(lo, hi);
+        fe[6 * f + 3] = tr[pair_index(pl, ph)];
+        fe[6 * f + 4] = max(mid, hi);
+        fe[6 * f + 5] = tr[pair_index(pm, ph)];
     }
 }
 
@@ -850,9 +863,9 @@ int ensure_fe(tg_mesh &m) {
     const int64_t F = m.n_faces;
     if ((rc = m.fe.ensure(sizeof(int32_t) * 6 * (size_t)std::max<int64_t>(F, 1)))) return rc;
     if (F > 0) {
-        hipLaunchKernelGGL(k_fe, dim3(grid(F)), dim3(256), 0, m.stream, m.fv.as<int32_t>(), F,
-                           m.vv_off.as<int32_t>(), m.vv_cols.as<int32_t>(),
-                           m.vv_eid.as<int32_t>(), m.fe.as<int32_t>());
+        hipLaunchKernelGGL(k_fe, dim3(grid(F)), dim3(256), 0, m.stream, m.tv.as<const int4>(),
+                           m.fv.as<int32_t>(), F, m.ft.as<int32_t>(), m.te.as<int32_t>(),
+                           m.fe.as<int32_t>());
         TG_LAUNCH_CHECK("k_fe");
     }
     m.fe_built = true;
