@@ -291,31 +291,36 @@ __device__ __forceinline__ int32_t face_first(const int4 *tv4, int64_t T, const 
     return best;
 }
 
+// Per slot position p = 4t + s: the first position of its face, and the
+// other cell holding it (-1 if none), so FT needs no second match.  A face
+// held by more than two cells sets fterr.
 __global__ void k_face_first(const int4 *__restrict__ tv4, int64_t T, const int32_t *__restrict__ te,
                              int64_t E, const int64_t *__restrict__ et_off, const int32_t *__restrict__ et,
                              int32_t *__restrict__ ffirst, uint8_t *__restrict__ flags,
-                             int32_t *__restrict__ err) {
+                             int32_t *__restrict__ other, int32_t *__restrict__ err,
+                             int32_t *__restrict__ fterr) {
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < 4 * T;
          p += (int64_t)gridDim.x * blockDim.x) {
-        int32_t nm;
-        const int32_t f = face_first(tv4, T, te, E, et_off, et, (int32_t)(p >> 2), (int)(p & 3),
-                                     nm, nullptr);
+        int32_t nm, two[2] = {-1, -1};
+        const int32_t t = (int32_t)(p >> 2);
+        const int32_t f = face_first(tv4, T, te, E, et_off, et, t, (int)(p & 3), nm, two);
         // the cell itself always matches; no match means ET and TE disagree
         const bool bad = f > (int32_t)p;
         if (bad) *err = 1;
+        if (nm > 2) *fterr = 1;  // every writer stores the same value
         ffirst[p] = bad ? (int32_t)p : f;
         flags[p] = bad || f == (int32_t)p;
+        other[p] = nm >= 2 ? (two[0] == t ? two[1] : two[0]) : -1;
     }
 }
 
 // TF; FV as (mid, hi, lo) -- the fields (middleVert, highVert, low vertex);
 // FT (<= 2 tets, ascending, -1 pad)
-__global__ void k_face_ids(const int4 *__restrict__ tv4, int64_t T, const int32_t *__restrict__ te,
-                           int64_t E, const int64_t *__restrict__ et_off, const int32_t *__restrict__ et,
-                           const int32_t *__restrict__ ffirst, const uint32_t *__restrict__ bits,
+__global__ void k_face_ids(const int4 *__restrict__ tv4, int64_t T,
+                           const int32_t *__restrict__ ffirst, const int32_t *__restrict__ other,
+                           const uint32_t *__restrict__ bits,
                            const int32_t *__restrict__ wpre, int32_t *__restrict__ tf,
-                           int32_t *__restrict__ fv, int32_t *__restrict__ ft,
-                           int32_t *__restrict__ err) {
+                           int32_t *__restrict__ fv, int32_t *__restrict__ ft) {
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < 4 * T;
          p += (int64_t)gridDim.x * blockDim.x) {
         const int32_t f = ffirst[p];
@@ -330,11 +335,9 @@ __global__ void k_face_ids(const int4 *__restrict__ tv4, int64_t T, const int32_
         fv[3 * (int64_t)id] = corner(q, mid);
         fv[3 * (int64_t)id + 1] = corner(q, hi);
         fv[3 * (int64_t)id + 2] = corner(q, lo);
-        int32_t nm, two[2] = {-1, -1};
-        face_first(tv4, T, te, E, et_off, et, t, s, nm, two);
-        if (nm > 2) *err = 1;  // every writer stores the same value
-        ft[2 * (int64_t)id] = two[0];
-        ft[2 * (int64_t)id + 1] = nm >= 2 ? two[1] : -1;
+        // the first position's cell is the smallest holding the face
+        ft[2 * (int64_t)id] = t;
+        ft[2 * (int64_t)id + 1] = other[p];
     }
 }
 
@@ -811,10 +814,13 @@ int ensure_faces(tg_mesh &m) {
     int32_t *err = m.counters.as<int32_t>() + C_FTERR;
     int32_t *iderr = m.counters.as<int32_t>() + C_IDERR;
     if ((rc = fill_i32(err, 0, 2, m.stream))) return rc;
+    if ((rc = m.tmp_c.ensure(I * std::max<size_t>(4 * (size_t)T, 1)))) return rc;  // other cell
+    int32_t *other = m.tmp_c.as<int32_t>();
     if (T > 0) {
         hipLaunchKernelGGL(k_face_first, dim3(grid(4 * T)), dim3(256), 0, m.stream,
                            m.tv.as<const int4>(), T, m.te.as<int32_t>(), m.n_edges,
-                           m.et_off.as<int64_t>(), m.et_vals.as<int32_t>(), ffirst, flags, iderr);
+                           m.et_off.as<int64_t>(), m.et_vals.as<int32_t>(), ffirst, flags, other,
+                           iderr, err);
         TG_LAUNCH_CHECK("k_face_first");
     }
     int32_t *bits, *wpre;
@@ -827,10 +833,8 @@ int ensure_faces(tg_mesh &m) {
         return rc;
     if (T > 0) {
         hipLaunchKernelGGL(k_face_ids, dim3(grid(4 * T)), dim3(256), 0, m.stream,
-                           m.tv.as<const int4>(), T, m.te.as<int32_t>(), m.n_edges,
-                           m.et_off.as<int64_t>(), m.et_vals.as<int32_t>(), ffirst,
-                           (const uint32_t *)bits, wpre,
-                           m.tf.as<int32_t>(), m.fv.as<int32_t>(), m.ft.as<int32_t>(), err);
+                           m.tv.as<const int4>(), T, ffirst, other, (const uint32_t *)bits, wpre,
+                           m.tf.as<int32_t>(), m.fv.as<int32_t>(), m.ft.as<int32_t>());
         TG_LAUNCH_CHECK("k_face_ids");
     }
     int32_t ferr = 0, ierr = 0;
