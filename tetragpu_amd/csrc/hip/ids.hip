@@ -138,37 +138,69 @@ __global__ void k_edge_ids(const int32_t *__restrict__ vv_off, const int32_t *__
     }
 }
 
-// VE rows {hi, id}: lane per vertex v walks its tets ascending; a pair whose
-// low vertex is v and whose position is a first occurrence mints the next id
-// of v's row, so rows come out in ascending id order.
-template <bool FILL>
-__global__ void k_ve(const int4 *__restrict__ tv4, const int32_t *__restrict__ vt_off,
-                     const int32_t *__restrict__ vtc, int64_t V, const uint8_t *__restrict__ flags,
-                     const uint32_t *__restrict__ bits, const int32_t *__restrict__ wpre,
-                     int32_t *__restrict__ cnt_or_off, int32_t *__restrict__ out) {
+// VE rows {hi, id}: the upper entries (u > v) of v's VV row, ordered by edge
+// id.  An edge's id ranks its first position, and the reference mints the
+// ids of v's row in the order v's cells first meet its upper neighbours, so
+// ordering by id is the reference's row order.  Rows are sorted in an LDS
+// column (insertion sort) when they fit, else in place in global memory.
+constexpr int VE_THREADS = 64, VE_CAP = 64;
+
+__global__ void k_ve_count(const int32_t *__restrict__ vv_off, const int32_t *__restrict__ vv,
+                           int64_t V, int32_t *__restrict__ cnt) {
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
          v += (int64_t)gridDim.x * blockDim.x) {
-        int32_t n = 0;
-        int32_t *row = FILL ? out + 2 * (int64_t)cnt_or_off[v] : nullptr;
-        for (int32_t k = vt_off[v]; k < vt_off[v + 1]; k++) {
-            const int32_t p = vtc[k];
-            const int32_t t = p >> 2, jv = p & 3;
-            const int4 q = tv4[t];
-            for (int i = 0; i < 3; i++)
-                for (int j = i + 1; j < 4; j++) {
-                    if (i != jv && j != jv) continue;
-                    const int32_t u = corner(q, i == jv ? j : i);
-                    if (u < v) continue;
-                    const int32_t pos = 6 * t + pair_index(i, j);
-                    if (!flags[pos]) continue;
-                    if (FILL) {
-                        row[2 * n] = u;
-                        row[2 * n + 1] = rank_of(bits, wpre, pos);
-                    }
-                    n++;
-                }
+        const int32_t rb = vv_off[v], rn = vv_off[v + 1] - rb;
+        // entries > v: the row is ascending and never holds v
+        int32_t lo = 0, hi = rn;
+        while (lo < hi) {
+            const int32_t mid = (lo + hi) >> 1;
+            if (vv[rb + mid] < (int32_t)v) lo = mid + 1;
+            else hi = mid;
         }
-        if (!FILL) cnt_or_off[v] = n;
+        cnt[v] = rn - lo;
+    }
+}
+
+__global__ __launch_bounds__(VE_THREADS) void k_ve_fill(const int32_t *__restrict__ vv_off,
+                                                        const int32_t *__restrict__ vv,
+                                                        const int32_t *__restrict__ vv_eid,
+                                                        int64_t V, const int32_t *__restrict__ off,
+                                                        int32_t *__restrict__ out) {
+    __shared__ int64_t col[VE_CAP][VE_THREADS];  // (id << 32) | hi
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t rb = vv_off[v], re = vv_off[v + 1];
+        const int32_t n = off[v + 1] - off[v];
+        int32_t *row = out + 2 * (int64_t)off[v];
+        const int32_t ub = re - n;  // first upper entry
+        if (n <= VE_CAP) {
+            for (int32_t k = 0; k < n; k++) {
+                const int64_t key = ((int64_t)vv_eid[ub + k] << 32) | (uint32_t)vv[ub + k];
+                int32_t p = k;
+                while (p > 0 && col[p - 1][threadIdx.x] > key) {
+                    col[p][threadIdx.x] = col[p - 1][threadIdx.x];
+                    p--;
+                }
+                col[p][threadIdx.x] = key;
+            }
+            for (int32_t k = 0; k < n; k++) {
+                const int64_t key = col[k][threadIdx.x];
+                row[2 * k] = (int32_t)(uint32_t)key;
+                row[2 * k + 1] = (int32_t)(key >> 32);
+            }
+        } else {
+            for (int32_t k = 0; k < n; k++) {
+                const int32_t id = vv_eid[ub + k], u = vv[ub + k];
+                int32_t p = k;
+                while (p > 0 && row[2 * (p - 1) + 1] > id) {
+                    row[2 * p] = row[2 * (p - 1)];
+                    row[2 * p + 1] = row[2 * (p - 1) + 1];
+                    p--;
+                }
+                row[2 * p] = u;
+                row[2 * p + 1] = id;
+            }
+        }
     }
 }
 
@@ -742,22 +774,21 @@ int ensure_edges(tg_mesh &m) {
     const int4 *tv4 = m.tv.as<const int4>();
     int32_t *voff = m.ve_off.as<int32_t>();
     if (V > 0) {
-        hipLaunchKernelGGL((k_ve<false>), dim3(grid(V)), dim3(256), 0, m.stream, tv4,
-                           m.vt_off.as<int32_t>(), m.vtc.as<int32_t>(), V, flags,
-                           (const uint32_t *)bits, wpre, voff, nullptr);
-        TG_LAUNCH_CHECK("k_ve count");
+        hipLaunchKernelGGL(k_ve_count, dim3(grid(V)), dim3(256), 0, m.stream,
+                           m.vv_off.as<int32_t>(), m.vv_cols.as<int32_t>(), V, voff);
+        TG_LAUNCH_CHECK("k_ve_count");
     }
     if ((rc = scan_exclusive_i32(voff, voff, V, m.scan_tmp.p, m.stream))) return rc;
     if (V > 0) {
-        hipLaunchKernelGGL((k_ve<true>), dim3(grid(V)), dim3(256), 0, m.stream, tv4,
-                           m.vt_off.as<int32_t>(), m.vtc.as<int32_t>(), V, flags,
-                           (const uint32_t *)bits, wpre, voff, m.ve_vals.as<int32_t>());
-        TG_LAUNCH_CHECK("k_ve fill");
         hipLaunchKernelGGL(k_edge_ids, dim3(grid(V)), dim3(256), 0, m.stream,
                            m.vv_off.as<int32_t>(), m.vv_cols.as<int32_t>(), V,
                            m.edge_first.as<int32_t>(), (const uint32_t *)bits, wpre,
                            m.vv_eid.as<int32_t>(), m.ev.as<int32_t>());
         TG_LAUNCH_CHECK("k_edge_ids");
+        hipLaunchKernelGGL(k_ve_fill, dim3(grid(V, VE_THREADS)), dim3(VE_THREADS), 0, m.stream,
+                           m.vv_off.as<int32_t>(), m.vv_cols.as<int32_t>(),
+                           m.vv_eid.as<int32_t>(), V, voff, m.ve_vals.as<int32_t>());
+        TG_LAUNCH_CHECK("k_ve_fill");
         hipLaunchKernelGGL(k_edge_lower, dim3(grid(V)), dim3(256), 0, m.stream,
                            m.vv_off.as<int32_t>(), m.vv_cols.as<int32_t>(), V,
                            m.vv_eid.as<int32_t>());
