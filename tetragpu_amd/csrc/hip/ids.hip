@@ -514,11 +514,11 @@ __global__ __launch_bounds__(ET_THREADS) void k_ef(const int32_t *__restrict__ e
 // memory latency instead of one per element; the merge then reads LDS.
 // A thread whose rows exceed ROW_CAP merges from global memory.
 constexpr int ROW_THREADS = 64;
-constexpr int ROW_CAP = 128;
+constexpr int ROW_CAP = 112;  // 28 KB per 64 threads: LDS bounds the resident waves
 
 __device__ __forceinline__ void stage_row(int32_t (*buf)[ROW_THREADS], int at,
                                           const int32_t *__restrict__ src, int32_t n, int shift) {
-#pragma unroll 4
+#pragma unroll 8
     for (int32_t k = 0; k < n; k++) buf[at + k][threadIdx.x] = src[k] >> shift;
 }
 
