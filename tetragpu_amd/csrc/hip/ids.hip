@@ -455,26 +455,23 @@ __device__ __forceinline__ void insert_sorted(int32_t *row, int64_t &m, int32_t 
 }
 
 // EF(e): faces holding e, from the two faces of e in each cell of ET(e).
-// The sorted, duplicate-free row is built in an LDS column when 2|ET(e)|
-// fits (else in global memory: tmp for the count pass, the row itself for
-// the fill pass).
+// One pass builds each sorted, duplicate-free row in a padded slot of
+// 2|ET(e)| words (tmp + 2 et_off[e]; in an LDS column while it fits) and
+// counts it; k_ef_copy then packs the rows after the scan of the counts.
 constexpr int EF_CAP = 64;
 
-template <bool FILL>
-__global__ __launch_bounds__(ET_THREADS) void k_ef(const int32_t *__restrict__ ev, int64_t E,
-                                                   const int64_t *__restrict__ et_off,
-                                                   const int32_t *__restrict__ et,
+__global__ __launch_bounds__(ET_THREADS) void k_ef(const int64_t *__restrict__ et_off,
+                                                   int64_t E, const int32_t *__restrict__ et,
                                                    const int32_t *__restrict__ te,
                                                    const int32_t *__restrict__ tf,
-                                                   int64_t *__restrict__ cnt_or_off,
-                                                   int32_t *__restrict__ out,
+                                                   int64_t *__restrict__ cnt,
                                                    int32_t *__restrict__ tmp) {
     __shared__ int32_t col[EF_CAP][ET_THREADS];
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
          e += (int64_t)gridDim.x * blockDim.x) {
         const int64_t b0 = et_off[e], b1 = et_off[e + 1];
         const bool lds = 2 * (b1 - b0) <= EF_CAP;
-        int32_t *grow = FILL ? out + cnt_or_off[e] : tmp + 2 * b0;
+        int32_t *grow = tmp + 2 * b0;
         int64_t m = 0;
         for (int64_t i = b0; i < b1; i++) {
             const int32_t t = et[i];
@@ -502,9 +499,21 @@ __global__ __launch_bounds__(ET_THREADS) void k_ef(const int32_t *__restrict__ e
                 }
             }
         }
-        if (FILL && lds)
+        if (lds)
             for (int64_t q = 0; q < m; q++) grow[q] = col[q][threadIdx.x];
-        if (!FILL) cnt_or_off[e] = m;
+        cnt[e] = m;
+    }
+}
+
+// packed rows from the padded ones (off: exclusive scan of the counts)
+__global__ void k_ef_copy(const int64_t *__restrict__ et_off, int64_t E,
+                          const int64_t *__restrict__ off, const int32_t *__restrict__ tmp,
+                          int32_t *__restrict__ out) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t *src = tmp + 2 * et_off[e];
+        const int64_t o = off[e], n = off[e + 1] - o;
+        for (int64_t q = 0; q < n; q++) out[o + q] = src[q];
     }
 }
 
@@ -522,12 +531,24 @@ __device__ __forceinline__ void stage_row(int32_t (*buf)[ROW_THREADS], int at,
     for (int32_t k = 0; k < n; k++) buf[at + k][threadIdx.x] = src[k] >> shift;
 }
 
-// TT(t): union of VT rows of t's corners, minus t (4-way merge).
-template <bool FILL>
+// TT(t): union of VT rows of t's corners, minus t (4-way merge), written in
+// one pass to a padded slot of |VT rows| - 4 words at pad[t], with its
+// length in cnt[t]; k_pack_rows packs the rows after the scan of cnt.
+__global__ void k_tt_bound(const int4 *__restrict__ tv4, int64_t T,
+                           const int32_t *__restrict__ vt_off, int64_t *__restrict__ ub) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < T;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int4 q = tv4[t];
+        ub[t] = (int64_t)(vt_off[q.x + 1] - vt_off[q.x]) + (vt_off[q.y + 1] - vt_off[q.y]) +
+                (vt_off[q.z + 1] - vt_off[q.z]) + (vt_off[q.w + 1] - vt_off[q.w]) - 4;
+    }
+}
+
 __global__ __launch_bounds__(ROW_THREADS) void k_tt(const int4 *__restrict__ tv4, int64_t T,
                                                     const int32_t *__restrict__ vt_off,
                                                     const int32_t *__restrict__ vtc,
-                                                    int64_t *__restrict__ cnt_or_off,
+                                                    const int64_t *__restrict__ pad,
+                                                    int64_t *__restrict__ cnt,
                                                     int32_t *__restrict__ out) {
     __shared__ int32_t buf[ROW_CAP][ROW_THREADS];
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < T;
@@ -536,7 +557,7 @@ __global__ __launch_bounds__(ROW_THREADS) void k_tt(const int4 *__restrict__ tv4
         int32_t p0 = vt_off[q.x], e0 = vt_off[q.x + 1], p1 = vt_off[q.y], e1 = vt_off[q.y + 1];
         int32_t p2 = vt_off[q.z], e2 = vt_off[q.z + 1], p3 = vt_off[q.w], e3 = vt_off[q.w + 1];
         int64_t n = 0;
-        int32_t *row = FILL ? out + cnt_or_off[t] : nullptr;
+        int32_t *row = out + pad[t];
         const int32_t n0 = e0 - p0, n1 = e1 - p1, n2 = e2 - p2, n3 = e3 - p3;
         if (n0 + n1 + n2 + n3 <= ROW_CAP) {
             stage_row(buf, 0, vtc + p0, n0, 2);
@@ -558,7 +579,7 @@ __global__ __launch_bounds__(ROW_THREADS) void k_tt(const int4 *__restrict__ tv4
                 if (h2 == mn) h2 = ++a2 < z2 ? buf[a2][threadIdx.x] : INF;
                 if (h3 == mn) h3 = ++a3 < z3 ? buf[a3][threadIdx.x] : INF;
                 if (mn == (int32_t)t) continue;
-                if (FILL) row[n] = mn;
+                row[n] = mn;
                 n++;
             }
         } else {
@@ -572,11 +593,39 @@ __global__ __launch_bounds__(ROW_THREADS) void k_tt(const int4 *__restrict__ tv4
                 if (h2 == mn) h2 = ++p2 < e2 ? vtc[p2] >> 2 : INF;
                 if (h3 == mn) h3 = ++p3 < e3 ? vtc[p3] >> 2 : INF;
                 if (mn == (int32_t)t) continue;
-                if (FILL) row[n] = mn;
+                row[n] = mn;
                 n++;
             }
         }
-        if (!FILL) cnt_or_off[t] = n;
+        cnt[t] = n;
+    }
+}
+
+// Pack padded rows: row r of length off[r+1] - off[r] moves from src +
+// pad[r] to dst + off[r].  A block takes 256 consecutive rows and copies its
+// contiguous output range element by element (a row found by bisection of
+// the block's row starts in LDS), so reads and writes are coalesced.
+__global__ __launch_bounds__(256) void k_pack_rows(const int64_t *__restrict__ pad,
+                                                   const int64_t *__restrict__ off, int64_t rows,
+                                                   const int32_t *__restrict__ src,
+                                                   int32_t *__restrict__ dst) {
+    __shared__ int64_t s_off[257], s_pad[256];
+    const int64_t r0 = (int64_t)blockIdx.x * 256;
+    const int nr = (int)min<int64_t>(256, rows - r0);
+    for (int i = threadIdx.x; i <= nr; i += 256) {
+        s_off[i] = off[r0 + i];
+        if (i < nr) s_pad[i] = pad[r0 + i];
+    }
+    __syncthreads();
+    const int64_t ob = s_off[0], oe = s_off[nr];
+    for (int64_t o = ob + threadIdx.x; o < oe; o += 256) {
+        int lo = 0, hi = nr - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_off[mid] <= o) lo = mid;
+            else hi = mid - 1;
+        }
+        dst[o] = src[s_pad[lo] + (o - s_off[lo])];
     }
 }
 
@@ -926,21 +975,19 @@ int ensure_ef(tg_mesh &m) {
         return rc;
     int64_t *off = m.ef_off.as<int64_t>();
     if (E > 0) {
-        hipLaunchKernelGGL((k_ef<false>), dim3(grid(E, ET_THREADS)), dim3(ET_THREADS), 0, m.stream,
-                           m.ev.as<int32_t>(), E, m.et_off.as<int64_t>(), m.et_vals.as<int32_t>(),
-                           m.te.as<int32_t>(), m.tf.as<int32_t>(), off, nullptr,
-                           m.tmp_c.as<int32_t>());
-        TG_LAUNCH_CHECK("k_ef count");
+        hipLaunchKernelGGL(k_ef, dim3(grid(E, ET_THREADS)), dim3(ET_THREADS), 0, m.stream,
+                           m.et_off.as<int64_t>(), E, m.et_vals.as<int32_t>(),
+                           m.te.as<int32_t>(), m.tf.as<int32_t>(), off, m.tmp_c.as<int32_t>());
+        TG_LAUNCH_CHECK("k_ef");
     }
     int64_t nnz = 0;
     if ((rc = scan_counts64(m, off, E, nnz))) return rc;
     if ((rc = m.ef_vals.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)))) return rc;
     if (E > 0) {
-        hipLaunchKernelGGL((k_ef<true>), dim3(grid(E, ET_THREADS)), dim3(ET_THREADS), 0, m.stream,
-                           m.ev.as<int32_t>(), E, m.et_off.as<int64_t>(), m.et_vals.as<int32_t>(),
-                           m.te.as<int32_t>(), m.tf.as<int32_t>(), off, m.ef_vals.as<int32_t>(),
-                           nullptr);
-        TG_LAUNCH_CHECK("k_ef fill");
+        hipLaunchKernelGGL(k_ef_copy, dim3(grid(E)), dim3(256), 0, m.stream,
+                           m.et_off.as<int64_t>(), E, off, m.tmp_c.as<int32_t>(),
+                           m.ef_vals.as<int32_t>());
+        TG_LAUNCH_CHECK("k_ef_copy");
     }
     m.ef_nnz = nnz;
     m.ef_built = true;
@@ -952,22 +999,32 @@ int ensure_tt(tg_mesh &m) {
     int rc;
     if ((rc = ensure_vt_sorted(m))) return rc;
     const int64_t T = m.n_cells;
-    if ((rc = m.tt_off.ensure(sizeof(int64_t) * (size_t)(T + 1)))) return rc;
-    int64_t *off = m.tt_off.as<int64_t>();
+    if ((rc = m.tt_off.ensure(sizeof(int64_t) * (size_t)(T + 1))) ||
+        (rc = m.tmp_b.ensure(sizeof(int64_t) * (size_t)(T + 1))))
+        return rc;
+    int64_t *off = m.tt_off.as<int64_t>(), *pad = m.tmp_b.as<int64_t>();
     if (T > 0) {
-        hipLaunchKernelGGL((k_tt<false>), dim3(grid(T, ROW_THREADS)), dim3(ROW_THREADS), 0, m.stream,
+        hipLaunchKernelGGL(k_tt_bound, dim3(grid(T)), dim3(256), 0, m.stream,
+                           m.tv.as<const int4>(), T, m.vt_off.as<int32_t>(), pad);
+        TG_LAUNCH_CHECK("k_tt_bound");
+    }
+    int64_t padded = 0;
+    if ((rc = scan_counts64(m, pad, T, padded))) return rc;
+    if ((rc = m.tmp_c.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(padded, 1)))) return rc;
+    if (T > 0) {
+        hipLaunchKernelGGL(k_tt, dim3(grid(T, ROW_THREADS)), dim3(ROW_THREADS), 0, m.stream,
                            m.tv.as<const int4>(), T, m.vt_off.as<int32_t>(), m.vtc.as<int32_t>(),
-                           off, nullptr);
-        TG_LAUNCH_CHECK("k_tt count");
+                           pad, off, m.tmp_c.as<int32_t>());
+        TG_LAUNCH_CHECK("k_tt");
     }
     int64_t nnz = 0;
     if ((rc = scan_counts64(m, off, T, nnz))) return rc;
     if ((rc = m.tt_vals.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)))) return rc;
     if (T > 0) {
-        hipLaunchKernelGGL((k_tt<true>), dim3(grid(T, ROW_THREADS)), dim3(ROW_THREADS), 0, m.stream,
-                           m.tv.as<const int4>(), T, m.vt_off.as<int32_t>(), m.vtc.as<int32_t>(),
-                           off, m.tt_vals.as<int32_t>());
-        TG_LAUNCH_CHECK("k_tt fill");
+        hipLaunchKernelGGL(k_pack_rows, dim3((unsigned)((T + 255) / 256)), dim3(256), 0,
+                           m.stream, pad, off, T, m.tmp_c.as<int32_t>(),
+                           m.tt_vals.as<int32_t>());
+        TG_LAUNCH_CHECK("k_pack_rows");
     }
     m.tt_nnz = nnz;
     m.tt_built = true;
