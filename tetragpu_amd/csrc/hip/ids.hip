@@ -229,14 +229,26 @@ __global__ void k_te(int32_t *__restrict__ te, int64_t n, const int32_t *__restr
 }
 
 // ---- ET = VT(lo) n VT(hi) -------------------------------------------------------
-// Both rows are staged in LDS when they fit (see stage_row), then merged.
+// Both rows are staged in LDS when they fit (see stage_row), then merged in
+// one pass into a padded slot of min(|VT(lo)|, |VT(hi)|) words at pad[e],
+// with the row length in cnt[e]; k_pack_rows packs the rows afterwards.
 constexpr int ET_THREADS = 64, ET_CAP = 128;
 
-template <bool FILL>
+__global__ void k_et_bound(const int32_t *__restrict__ ev, int64_t E, int64_t V,
+                           const int32_t *__restrict__ vt_off, int64_t *__restrict__ ub) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t a = ev[2 * e], b = ev[2 * e + 1];
+        const bool ok = a >= 0 && a < V && b >= 0 && b < V;  // always, for a built EV
+        ub[e] = ok ? min(vt_off[a + 1] - vt_off[a], vt_off[b + 1] - vt_off[b]) : 0;
+    }
+}
+
 __global__ __launch_bounds__(ET_THREADS) void k_et(const int32_t *__restrict__ ev, int64_t E,
                                                    int64_t V, const int32_t *__restrict__ vt_off,
                                                    const int32_t *__restrict__ vtc,
-                                                   int64_t *__restrict__ cnt_or_off,
+                                                   const int64_t *__restrict__ pad,
+                                                   int64_t *__restrict__ cnt,
                                                    int32_t *__restrict__ out) {
     __shared__ int32_t buf[ET_CAP][ET_THREADS];
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
@@ -245,7 +257,8 @@ __global__ __launch_bounds__(ET_THREADS) void k_et(const int32_t *__restrict__ e
         const bool ok = a >= 0 && a < V && b >= 0 && b < V;  // always, for a built EV
         int32_t i = ok ? vt_off[a] : 0, ie = ok ? vt_off[a + 1] : 0;
         int32_t j = ok ? vt_off[b] : 0, je = ok ? vt_off[b + 1] : 0;
-        int64_t n = 0, o = FILL ? cnt_or_off[e] : 0;
+        int64_t n = 0;
+        int32_t *row = out + pad[e];
         const int32_t na = ie - i, nb = je - j;
         if (na + nb <= ET_CAP) {
 #pragma unroll 4
@@ -256,7 +269,7 @@ __global__ __launch_bounds__(ET_THREADS) void k_et(const int32_t *__restrict__ e
             int32_t x = i < ie ? buf[i][threadIdx.x] : INF, y = j < je ? buf[j][threadIdx.x] : INF;
             while (x != INF && y != INF) {
                 if (x == y) {
-                    if (FILL) out[o + n] = x;
+                    row[n] = x;
                     n++;
                 }
                 const int32_t m = min(x, y);
@@ -267,7 +280,7 @@ __global__ __launch_bounds__(ET_THREADS) void k_et(const int32_t *__restrict__ e
             int32_t x = i < ie ? vtc[i] >> 2 : INF, y = j < je ? vtc[j] >> 2 : INF;
             while (x != INF && y != INF) {
                 if (x == y) {
-                    if (FILL) out[o + n] = x;
+                    row[n] = x;
                     n++;
                 }
                 const int32_t m = min(x, y);
@@ -275,7 +288,7 @@ __global__ __launch_bounds__(ET_THREADS) void k_et(const int32_t *__restrict__ e
                 if (y == m) y = ++j < je ? vtc[j] >> 2 : INF;
             }
         }
-        if (!FILL) cnt_or_off[e] = n;
+        cnt[e] = n;
     }
 }
 
@@ -858,22 +871,31 @@ int ensure_et(tg_mesh &m) {
     // the vertex-lane VV form rewrites the corner lists unsorted: sort again
     if ((rc = ensure_edges(m)) || (rc = ensure_vt_sorted(m))) return rc;
     const int64_t E = m.n_edges;
-    if ((rc = m.et_off.ensure(sizeof(int64_t) * (size_t)(E + 1)))) return rc;
-    int64_t *off = m.et_off.as<int64_t>();
+    if ((rc = m.et_off.ensure(sizeof(int64_t) * (size_t)(E + 1))) ||
+        (rc = m.tmp_b.ensure(sizeof(int64_t) * (size_t)(E + 1))))
+        return rc;
+    int64_t *off = m.et_off.as<int64_t>(), *pad = m.tmp_b.as<int64_t>();
     if (E > 0) {
-        hipLaunchKernelGGL((k_et<false>), dim3(grid(E, ET_THREADS)), dim3(ET_THREADS), 0, m.stream,
+        hipLaunchKernelGGL(k_et_bound, dim3(grid(E)), dim3(256), 0, m.stream, m.ev.as<int32_t>(),
+                           E, m.n_points, m.vt_off.as<int32_t>(), pad);
+        TG_LAUNCH_CHECK("k_et_bound");
+    }
+    int64_t padded = 0;
+    if ((rc = scan_counts64(m, pad, E, padded))) return rc;
+    if ((rc = m.tmp_c.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(padded, 1)))) return rc;
+    if (E > 0) {
+        hipLaunchKernelGGL(k_et, dim3(grid(E, ET_THREADS)), dim3(ET_THREADS), 0, m.stream,
                            m.ev.as<int32_t>(), E, m.n_points, m.vt_off.as<int32_t>(),
-                           m.vtc.as<int32_t>(), off, nullptr);
-        TG_LAUNCH_CHECK("k_et count");
+                           m.vtc.as<int32_t>(), pad, off, m.tmp_c.as<int32_t>());
+        TG_LAUNCH_CHECK("k_et");
     }
     int64_t nnz = 0;
     if ((rc = scan_counts64(m, off, E, nnz))) return rc;
     if ((rc = m.et_vals.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)))) return rc;
     if (E > 0) {
-        hipLaunchKernelGGL((k_et<true>), dim3(grid(E, ET_THREADS)), dim3(ET_THREADS), 0, m.stream,
-                           m.ev.as<int32_t>(), E, m.n_points, m.vt_off.as<int32_t>(),
-                           m.vtc.as<int32_t>(), off, m.et_vals.as<int32_t>());
-        TG_LAUNCH_CHECK("k_et fill");
+        hipLaunchKernelGGL(k_pack_rows, dim3((unsigned)((E + 255) / 256)), dim3(256), 0,
+                           m.stream, pad, off, E, m.tmp_c.as<int32_t>(), m.et_vals.as<int32_t>());
+        TG_LAUNCH_CHECK("k_pack_rows");
     }
     m.et_built = true;
     return OK;
