@@ -285,6 +285,32 @@ def bench_irregular(t, torch, steps: int, seed: int = 5, check: bool = True) -> 
     return out
 
 
+def bench_partitioned(t, mesh, whole, n_parts: int = 8) -> dict:
+    """The partition loop (src/alg/critPoints.cu:654-1003) on the benchmark
+    mesh cut into n_parts slabs of equal z extent.  Each slab is classified on
+    the cells that touch it (a one-cell halo), one slab at a time on the GPU,
+    so only the largest slab's mesh is resident.  Kuhn grids are flag
+    complexes, so the merged classes must equal the whole-mesh run.  Times are
+    host wall clock: localization on the host, then per slab upload + VV +
+    classes + copy back (second of two runs)."""
+    import numpy as np
+
+    z = mesh.positions[:, 2].astype(np.float64)
+    lo, hi = float(z.min()), float(z.max())
+    parts = np.minimum(((z - lo) / max(hi - lo, 1e-30) * n_parts).astype(np.int32), n_parts - 1)
+    sc = mesh.scalars.astype(np.float64)
+    for _ in range(2):
+        cls, st = t.partitioned_classes(mesh.cells, mesh.n_points, sc, parts)
+    out = {"partitions": st["partitions"], "rule": "z slabs, cells touching the slab",
+           "total_s": round(st["total_s"], 3), "localize_s": round(st["localize_s"], 3),
+           "classify_s": round(st["classify_s"], 3), "largest_slab_cells": st["max_cells"],
+           "cells_with_halos": st["cells_total"],
+           "halo_overhead": round(st["cells_total"] / max(mesh.n_cells, 1) - 1.0, 4),
+           "identical_to_whole_mesh": bool(np.array_equal(cls, whole))}
+    log(f"partitioned: {out}")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -302,6 +328,8 @@ def main() -> int:
                     help="skip the per-relation build times on the large mesh")
     ap.add_argument("--no-irregular", action="store_true",
                     help="skip the irregular-degree mesh (Kuhn 96^3, 15 %% split)")
+    ap.add_argument("--no-partitions", action="store_true",
+                    help="skip the partition loop (8 z slabs of the benchmark mesh)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (reference kernel for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -403,9 +431,11 @@ def main() -> int:
     d2 = t.Mesh(mesh.cells, V, mesh.scalars, stream=stream.cuda_stream)
     d2.vv_critpoints()
     d2.relation("VV")
-    d2.classes()
+    whole_cls = d2.classes()
     e2e_ms = (time.perf_counter() - t0) * 1e3
     d2.close()
+
+    partitioned = None if args.no_partitions else bench_partitioned(t, mesh, whole_cls)
 
     relations = None if args.no_relations else bench_relations(t, torch, mesh)
     irregular = None if args.no_irregular else bench_irregular(t, torch, max(args.steps // 2, 1),
@@ -435,6 +465,7 @@ def main() -> int:
         "end_to_end_note": "host cells/scalars in, VV CSR + classes out, incl. allocation (first use)",
         "relations": relations,
         "irregular": irregular,
+        "partitioned": partitioned,
         "batched": batch,
         "serving": serve,
     }

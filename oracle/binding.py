@@ -224,6 +224,35 @@ def critpoints(VV: Csr, scalars, n_points) -> np.ndarray:
     return out
 
 
+def critpoints_partitioned(cells, n_points, scalars, parts, part_cells=None) -> np.ndarray:
+    """classes (V, 3) of the reference's partition loop: partition p is
+    classified on its cells -- part_cells[p] (a 0/1 row per cell) when given
+    (src/alg/critPoints.cu:731-748), else every cell with a vertex in p
+    (:814-838) -- with vertices renumbered in ascending global order
+    (:846-905); only p's own vertices keep their classes (:940-943) and
+    results merge by ownership (:1462-1491).  Vertices in no cell of their
+    partition stay (0, 0, 0)."""
+    cells = _i64(cells).reshape(-1, 4)
+    parts = np.ascontiguousarray(parts, dtype=np.int64)
+    sc = np.ascontiguousarray(scalars, dtype=np.float64)
+    out = np.zeros((n_points, 3), np.uint32)
+    n_parts = int(parts.max()) + 1 if parts.size else 1
+    for p in range(n_parts):
+        if part_cells is not None:
+            mask = np.asarray(part_cells[p]) == 1
+        else:
+            mask = (parts[cells] == p).any(axis=1) if cells.size else np.zeros(0, bool)
+        sub = cells[mask]
+        if sub.size == 0:
+            continue
+        verts = np.unique(sub)
+        local = np.searchsorted(verts, sub)
+        cls = critpoints(vv(local, verts.size), sc[verts], verts.size)
+        own = parts[verts] == p
+        out[verts[own]] = cls[own]
+    return out
+
+
 def label6(classes) -> np.ndarray:
     cl = np.ascontiguousarray(classes, dtype=np.uint32)
     out = np.empty(cl.shape[0], np.uint8)

@@ -5,7 +5,10 @@
 //   dropin_probe cpu <cells.i64> <n_points> [<scalars.f64>]
 //   dropin_probe parse <args...>
 //   dropin_probe export <classes.u32> <n_points> <debug>
-//   dropin_probe load <input> [<arrayname>]
+//   dropin_probe load <input> [<arrayname> [<partitioningname>]]
+//   dropin_probe part <cells.i64> <n_points> <scalars.f64> <parts.i32> <cpu|gpu>
+//                [<partcells.i32>]
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -140,6 +143,7 @@ int load_cmd(int argc, char **argv) {
     args.debug = NO_DEBUG;
     args.fileName = argv[2];
     if (argc > 3) args.arrayname = argv[3];
+    if (argc > 4) args.partitioningname = argv[4];
     auto tv = get_TV_from_VTK(args);
     std::cout << "{\"n_points\": " << tv->nPoints << ", \"n_cells\": " << tv->nCells
               << ", \"cells\": [";
@@ -151,7 +155,50 @@ int load_cmd(int argc, char **argv) {
             std::snprintf(buf, sizeof buf, "%.17g", tv->vertexAttributes[v]);
             std::cout << (v ? "," : "") << buf;
         }
+    std::cout << "], \"n_partitions\": " << tv->n_partitions << ", \"partition_ids\": [";
+    for (vtkIdType v = 0; v < tv->nPoints; v++) std::cout << (v ? "," : "") << tv->partitionIDs[v];
+    std::cout << "], \"n_per_partition\": [";
+    for (int p = 0; p < tv->n_partitions; p++) std::cout << (p ? "," : "") << tv->n_per_partition[p];
+    std::cout << "], \"partition_cells\": [";
+    if (tv->partitionCells)
+        for (vtkIdType i = 0; i < (vtkIdType)tv->n_partitions * tv->nCells; i++)
+            std::cout << (i ? "," : "") << tv->partitionCells[i];
     std::cout << "]}\n";
+    return 0;
+}
+
+// partitioned critical points through the C++ API (host or device builders)
+int part_cmd(int argc, char **argv) {
+    auto cells = read_file<int64_t>(argv[2]);
+    const vtkIdType V = std::atoll(argv[3]);
+    auto sc = read_file<double>(argv[4]);
+    auto parts = read_file<int32_t>(argv[5]);
+    const bool gpu = std::string(argv[6]) == "gpu";
+    TV_Data tv(V, (vtkIdType)cells.size() / 4);
+    std::memcpy(tv.cells, cells.data(), cells.size() * sizeof(int64_t));
+    tv.vertexAttributes = new double[V];
+    std::memcpy(tv.vertexAttributes, sc.data(), V * sizeof(double));
+    int np = 1;
+    for (int32_t p : parts) np = std::max(np, p + 1);
+    tv.n_partitions = np;
+    tv.partitionIDs = new int[V];
+    tv.n_per_partition = new int[np]();
+    for (vtkIdType v = 0; v < V; v++) {
+        tv.partitionIDs[v] = parts[v];
+        tv.n_per_partition[parts[v]]++;
+    }
+    if (argc > 7) {
+        auto pc = read_file<int32_t>(argv[7]);
+        tv.partitionCells = new int[pc.size()];
+        std::memcpy(tv.partitionCells, pc.data(), pc.size() * sizeof(int32_t));
+    }
+    runtime_arguments args;
+    args.debug = NO_DEBUG;
+    unsigned int *cls = partitioned_critical_points(tv, args, gpu);
+    std::cout << "{\"classes\": [";
+    for (vtkIdType i = 0; i < 3 * V; i++) std::cout << (i ? "," : "") << cls[i];
+    std::cout << "]}\n";
+    delete[] cls;
     return 0;
 }
 
@@ -164,5 +211,6 @@ int main(int argc, char **argv) {
     if (cmd == "parse") return parse_cmd(argc, argv);
     if (cmd == "export") return export_cmd(argv);
     if (cmd == "load") return load_cmd(argc, argv);
+    if (cmd == "part") return part_cmd(argc, argv);
     return 2;
 }

@@ -161,7 +161,7 @@ VTK_TYPES = {np.dtype(np.int32): "Int32", np.dtype(np.int64): "Int64",
 
 
 def write_vtu(path, cells, pts, arrays, fmt="ascii", compress=False, header="UInt32",
-              encoding="raw"):
+              encoding="raw", cell_arrays=()):
     hdt = "I" if header == "UInt32" else "Q"
     appended = bytearray()
 
@@ -198,7 +198,9 @@ def write_vtu(path, cells, pts, arrays, fmt="ascii", compress=False, header="UIn
 
     T, V = len(cells), len(pts)
     pd = "".join(data_array(n, a) for n, a in arrays)
-    body = (f'<Piece NumberOfPoints="{V}" NumberOfCells="{T}"><PointData>{pd}</PointData>'
+    cd = "".join(data_array(n, a) for n, a in cell_arrays)
+    cd = f"<CellData>{cd}</CellData>" if cd else ""
+    body = (f'<Piece NumberOfPoints="{V}" NumberOfCells="{T}"><PointData>{pd}</PointData>{cd}'
             f'<Points>{data_array("Points", pts.astype(np.float32), 3)}</Points><Cells>'
             f'{data_array("connectivity", cells.astype(np.int64).reshape(-1))}'
             f'{data_array("offsets", (4 * np.arange(1, T + 1)).astype(np.int64))}'
@@ -249,3 +251,102 @@ def test_kuhn_spec_input(probe):
     m = t.kuhn(3, seed=7, permute_vertices=False, jitter=False, scalar="linear")
     assert j["cells"] == m.cells.reshape(-1).tolist()
     assert j["scalars"] == m.scalars.astype(np.float64).tolist()
+
+
+# ---- partitions (src/vtk_load.cu:148-204, src/alg/critPoints.cu:654-1003) -------------
+@pytest.mark.parametrize("raw,want", [
+    ([1, 2, 2, 3, 1], [0, 1, 1, 2, 0]),        # 1-based ids are lowered
+    ([1, 0, 2, 1], [1, 0, 2, 1]),              # a 0 switches to 0-based, earlier ids raised back
+    ([2, 0, 1, 0], [2, 0, 1, 0]),
+    ([1, 0, -1, 2], [1, 1, 0, 2]),             # a later negative value moves the switch point
+    ([1.9, 2.2, 1.0], [0, 1, 0]),              # fractions truncate
+])
+def test_vtu_partition_reindexing(probe, tmp_path, raw, want):
+    V = len(raw)
+    cells = np.array([[0, 1, 2, 3]]) if V >= 4 else np.array([[0, 1, 2, 0]])[:0]
+    path = tmp_path / "p.vtu"
+    write_vtu(path, cells, np.zeros((V, 3)), [("s", np.arange(V, dtype=np.float64)),
+                                               ("part", np.array(raw, np.float64))])
+    j = last_json(run(probe, "load", path, "s", "part"))
+    assert j["partition_ids"] == want
+    assert j["n_partitions"] == len(set(want))
+    assert j["n_per_partition"] == [want.count(p) for p in range(len(set(want)))]
+    assert j["partition_cells"] == []
+
+
+def test_vtu_partition_errors(probe, tmp_path):
+    path = tmp_path / "p.vtu"
+    write_vtu(path, np.array([[0, 1, 2, 3]]), np.zeros((4, 3)),
+              [("s", np.arange(4.0)), ("part", np.array([1.0, 1.0, 6.0, 6.0]))])
+    r = run(probe, "load", path, "s", "part", check=False)
+    assert r.returncode == 1 and "not 0..1" in r.stderr
+    r = run(probe, "load", path, "s", "missing", check=False)
+    assert r.returncode == 1 and "Unable to find partitioning name 'missing'" in r.stderr
+
+
+def test_vtu_default_single_partition(probe, tmp_path):
+    j = last_json(run(probe, "load", "kuhn:2:1"))
+    assert j["n_partitions"] == 1 and j["partition_ids"] == [0] * j["n_points"]
+    assert j["n_per_partition"] == [j["n_points"]]
+
+
+def test_vtu_partition_cell_lists(probe, tmp_path):
+    m = t.kuhn(2, seed=3)
+    parts = (np.arange(m.n_points) % 2).astype(np.int32)
+    rng = np.random.default_rng(1)
+    lists = [rng.integers(0, 2, m.n_cells).astype(np.int32) for _ in range(2)]
+    path = tmp_path / "p.vtu"
+    write_vtu(path, m.cells, np.zeros((m.n_points, 3)),
+              [("s", m.scalars.astype(np.float64)), ("part", parts + 1)],
+              cell_arrays=[("partition_cells_1", lists[1]), ("partition_cells_0", lists[0])])
+    j = last_json(run(probe, "load", path, "s", "part"))
+    assert j["partition_ids"] == parts.tolist()
+    assert j["partition_cells"] == np.concatenate(lists).tolist()
+    # one list missing: the vertex rule applies (no lists kept)
+    write_vtu(path, m.cells, np.zeros((m.n_points, 3)),
+              [("s", m.scalars.astype(np.float64)), ("part", parts + 1)],
+              cell_arrays=[("partition_cells_1", lists[1])])
+    j = last_json(run(probe, "load", path, "s", "part"))
+    assert j["partition_cells"] == []
+
+
+def probe_part(probe, tmp_path, cells, V, s, parts, backend, part_cells=None):
+    files = []
+    for name, a, dt in (("c.i64", cells, np.int64), ("s.f64", s, np.float64),
+                        ("p.i32", parts, np.int32)):
+        f = tmp_path / name
+        np.ascontiguousarray(a, dt).tofile(f)
+        files.append(f)
+    args = ["part", files[0], V, files[1], files[2], backend]
+    if part_cells is not None:
+        f = tmp_path / "pc.i32"
+        np.ascontiguousarray(part_cells, np.int32).tofile(f)
+        args.append(f)
+    return np.array(last_json(run(probe, *args))["classes"], np.uint32).reshape(-1, 3)
+
+
+def partition_cases():
+    rng = np.random.default_rng(8)
+    soup = np.array([rng.choice(50, 4, replace=False) for _ in range(160)])
+    s_soup = rng.integers(0, 5, 50).astype(np.float64)
+    k = t.kuhn(5, seed=6, scalar="gaussians")
+    hub = t.kuhn(4, seed=2, split_fraction=0.3, n_hubs=3, hub_splits=20, scalar="sines")
+    nf = np.array([[0, 1, 2, 3], [0, 4, 5, 6], [1, 4, 7, 8]])
+    return [
+        ("soup4", soup, 50, s_soup, np.arange(50) % 4, None),
+        ("soup_lists", soup, 50, s_soup, np.arange(50) % 3,
+         rng.integers(0, 2, (3, len(soup)))),
+        ("kuhn5", k.cells, k.n_points, k.scalars.astype(np.float64),
+         rng.integers(0, 6, k.n_points), None),
+        ("hubs", hub.cells, hub.n_points, hub.scalars.astype(np.float64),
+         rng.integers(0, 3, hub.n_points), None),
+        ("non_flag", nf, 10, np.array([10, 1, 2, 3, 4, 5, 6, 20, 20, 0], np.float64),
+         np.array([0, 1, 1, 1, 1, 1, 1, 1, 1, 1]), None),
+    ]
+
+
+@pytest.mark.parametrize("case", partition_cases(), ids=lambda c: c[0])
+def test_cpu_partitioned_critpoints_equal_oracle(probe, tmp_path, case):
+    _, cells, V, s, parts, pc = case
+    got = probe_part(probe, tmp_path, cells, V, s, parts, "cpu", pc)
+    assert np.array_equal(got, O.critpoints_partitioned(cells, V, s, parts, pc))

@@ -70,3 +70,39 @@ def test_vtu_input_default_flags(gpu, tmp_path):
 def test_bad_input_exits_with_failure(gpu, tmp_path):
     r = tetra_main("-i", tmp_path / "missing.vtu")
     assert r.returncode == 1 and "cannot open" in r.stderr
+
+
+# ---- partition loop on the device (src/alg/critPoints.cu:654-1003, 1438-1491) -----------
+from test_dropin_cpu import partition_cases, probe, probe_part  # noqa: E402,F401
+
+
+@pytest.mark.parametrize("case", partition_cases(), ids=lambda c: c[0])
+def test_gpu_partitioned_critpoints_equal_oracle(gpu, probe, tmp_path, case):  # noqa: F811
+    name, cells, V, s, parts, pc = case
+    print("partitioned", name, flush=True)
+    got = probe_part(probe, tmp_path, cells, V, s, parts, "gpu", pc)
+    assert np.array_equal(got, O.critpoints_partitioned(cells, V, s, parts, pc))
+
+
+def test_tetra_main_partitions_from_vtu(gpu, tmp_path):
+    from test_dropin_cpu import write_vtu
+    import tetragpu_amd as t
+
+    m = t.kuhn(6, seed=12, split_fraction=0.2, scalar="gaussians")
+    parts = np.random.default_rng(5).integers(0, 4, m.n_points)
+    path = tmp_path / "m.vtu"
+    write_vtu(path, m.cells, np.zeros((m.n_points, 3)),
+              [("f", m.scalars), ("part", (parts + 1).astype(np.int32))], "binary", True)
+    out = tmp_path / "classes.txt"
+    r = tetra_main("-i", path, "-a", "f", "-p", "part", "-e", out, "-d", "2")
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "[validate] all checks passed" in r.stderr
+    cls = O.critpoints_partitioned(m.cells, m.n_points, m.scalars.astype(np.float64), parts)
+    text = out.read_text()
+    tot = totals(text)
+    assert tot["minima"] == (cls[:, 2] == 2).sum() and tot["maxima"] == (cls[:, 2] == 1).sum()
+    assert tot["saddles"] == (cls[:, 2] == 4).sum() and tot["regular"] == (cls[:, 2] == 3).sum()
+    for p in range(4):
+        line = f"Partition {p} number of saddles:"
+        got = [int(l.split(":")[1]) for l in text.splitlines() if l.startswith(line)]
+        assert got == [int(((cls[:, 2] == 4) & (parts == p)).sum())]

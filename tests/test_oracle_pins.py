@@ -276,3 +276,60 @@ def test_omp_matches_serial():
     ref_vv = O.vv(m.cells, m.n_points).sorted_rows()
     assert np.array_equal(VV.off, ref_vv.off) and np.array_equal(VV.val, ref_vv.val)
     assert np.array_equal(cls, O.critpoints(ref_vv, m.scalars.astype(np.float64), m.n_points))
+
+
+# ---- partition loop (src/alg/critPoints.cu:654-1003, 1438-1491) -------------------------
+def _soup(seed, V, T):
+    rng = np.random.default_rng(seed)
+    return np.array([rng.choice(V, 4, replace=False) for _ in range(T)])
+
+
+def test_partitioned_one_partition_is_whole_mesh_but_voids():
+    cells = np.array([[0, 1, 2, 3], [1, 2, 3, 5]])
+    s = np.array([3.0, 1.0, 4.0, 1.0, 5.0, 9.0])
+    whole = O.critpoints(O.vv(cells, 6), s, 6)
+    part = O.critpoints_partitioned(cells, 6, s, np.zeros(6, np.int64))
+    # vertex 4 is in no cell: the whole-mesh run classifies it (0, 0, 4),
+    # the partition loop never sees it
+    assert whole[4].tolist() == [0, 0, 4] and part[4].tolist() == [0, 0, 0]
+    keep = np.arange(6) != 4
+    assert np.array_equal(whole[keep], part[keep])
+
+
+@pytest.mark.parametrize("n_parts", [2, 5, 17])
+def test_partitioned_flag_mesh_equals_whole(n_parts):
+    # Kuhn grids are flag complexes: any two neighbours of v that are joined
+    # by an edge span a triangle with v, held by a cell of v's partition
+    m = t.kuhn(6, seed=n_parts, scalar="gaussians")
+    parts = np.random.default_rng(n_parts).integers(0, n_parts, m.n_points)
+    s = m.scalars.astype(np.float64)
+    whole = O.critpoints(O.vv(m.cells, m.n_points), s, m.n_points)
+    assert np.array_equal(O.critpoints_partitioned(m.cells, m.n_points, s, parts), whole)
+
+
+def test_partitioned_non_flag_known_answer():
+    # the cell (1, 4, 7, 8) that joins vertex 0's two lower groups has no
+    # vertex of partition 0, so partition 0 sees two lower components
+    cells = np.array([[0, 1, 2, 3], [0, 4, 5, 6], [1, 4, 7, 8]])
+    s = np.array([10, 1, 2, 3, 4, 5, 6, 20, 20], np.float64)
+    parts = np.array([0, 1, 1, 1, 1, 1, 1, 1, 1])
+    cls = O.critpoints_partitioned(cells, 9, s, parts)
+    assert cls[0].tolist() == [2, 0, 2]
+    # partition 1 holds every cell: its vertices match the whole mesh
+    whole = O.critpoints(O.vv(cells, 9), s, 9)
+    assert np.array_equal(cls[1:], whole[1:])
+    # with vertex 1 or 4 in partition 0 the joining cell is included again
+    parts[4] = 0
+    assert O.critpoints_partitioned(cells, 9, s, parts)[0].tolist() == [1, 0, 2]
+
+
+def test_partitioned_cell_lists_of_all_cells_equal_whole():
+    cells = _soup(4, 30, 80)
+    s = np.random.default_rng(4).integers(0, 6, 30).astype(np.float64)
+    parts = np.arange(30) % 3
+    full = np.ones((3, len(cells)), np.int32)
+    whole = O.critpoints(O.vv(cells, 30), s, 30)
+    got = O.critpoints_partitioned(cells, 30, s, parts, full)
+    used = np.zeros(30, bool)
+    used[np.unique(cells)] = True
+    assert np.array_equal(got[used], whole[used]) and not got[~used].any()

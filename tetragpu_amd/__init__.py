@@ -5,8 +5,9 @@ Native code: tetragpu_amd/lib/libtetra_hip.so (HIP kernels + C ABI,
 include/tetra_c.h) and libtetra_host.so (mesh generator, C++ drop-in layer).
 """
 from .batch import Batch
-from .mesh import ALL_FLAGS, CLASS_NAMES, LABEL6_NAMES, REL, Mesh, device_count, flags, init
+from .mesh import (ALL_FLAGS, CLASS_NAMES, LABEL6_NAMES, REL, Mesh, device_count, flags, init,
+                   partitioned_classes)
 from .meshgen import TetMesh, kuhn, kuhn_counts
 
 __all__ = ["Batch", "Mesh", "REL", "ALL_FLAGS", "CLASS_NAMES", "LABEL6_NAMES", "flags", "init",
-           "device_count", "TetMesh", "kuhn", "kuhn_counts"]
+           "device_count", "partitioned_classes", "TetMesh", "kuhn", "kuhn_counts"]

@@ -87,6 +87,8 @@ _HOST_SIGS = {
     "tg_autotune": (C.c_int, [P, I32, I32, P, I32, C.POINTER(I32)]),
     "tg_tuning_load": (C.c_int, [C.c_char_p]),
     "tg_tuning_save": (C.c_int, [C.c_char_p]),
+    "tetra_partitioned_classes": (C.c_int, [P, C.c_int64, C.c_int64, P, P, P, P]),
+    "tetra_partition_error": (C.c_char_p, []),
 }
 
 

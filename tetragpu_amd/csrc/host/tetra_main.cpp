@@ -178,7 +178,13 @@ int main(int argc, char *argv[]) {
     if (TV->vertexAttributes) {
         unsigned int *classes = gpu_critical_points(*TV, args);
         say(args, "critical points [GPU]", clk.lap());
-        if (check) {
+        if (check && !args.no_partitioning && TV->partitionIDs) {
+            // the same partition loop with the host builders
+            unsigned int *h = partitioned_critical_points(*TV, args, false);
+            say(args, "critical points [CPU, partitions]", clk.lap());
+            ok &= check_exact_classes(h, classes, nP);
+            delete[] h;
+        } else if (check) {
             if (!hVV) hVV = elective_make_VV(*TV, nP);
             std::vector<unsigned int> h(3 * (size_t)nP);
             cpu_critpoints(*hVV, TV->vertexAttributes, nP, h.data(), args.threadNumber);

@@ -8,8 +8,9 @@
 // types Int8..UInt64, Float32, Float64.  Cells must all be tetrahedra
 // (connectivity with offsets 4, 8, 12, ...).  The point array named by
 // --arrayname, else the first point array, becomes the scalar field (the
-// reference's choice, src/vtk_load.cu:131-143); --partitioningname is not
-// used (one GPU, no partitions).
+// reference's choice, src/vtk_load.cu:131-143).  --partitioningname names
+// the point array of partition ids (re-indexed as the reference does), and
+// cell arrays partition_cells_<p> give per-partition cell lists.
 #include "vtk_load.h"
 
 #include <zlib.h>
@@ -74,6 +75,60 @@ struct Tag {
     size_t body_begin = 0, body_end = 0;  // text between <tag ...> and </tag>
     bool self_closing = false;
 };
+
+// Partition ids from a point array (src/vtk_load.cu:171-204).
+// - Values are read as 1-based and lowered by one, until the first value
+//   that would go negative.
+// - That value becomes 0. Later values are kept as they are. The ids
+//   already lowered, from the point before it back to the start, are raised
+//   by one again.
+// - A later negative value moves that point, so every id before it ends up
+//   raised by one. This is the reference's behaviour and is kept.
+// - Fractions truncate toward zero.
+// - The ids must then be 0..n-1, where n is the number of distinct values.
+//   The reference indexes its per-partition counts with them unchecked.
+void assign_partitions(TV_Data &tv, const std::vector<double> &val) {
+    const int64_t n = tv.nPoints;
+    std::vector<int> ids((size_t)n);
+    int64_t reindex_from = n;
+    for (int64_t i = 0; i < n; i++) {
+        const double x = val[i] - (reindex_from == n ? 1.0 : 0.0);
+        if (!(x > -2147483648.0 && x < 2147483648.0))
+            fail("partition id of point " + std::to_string(i) + " is not a 32-bit number");
+        ids[i] = (int)x;
+        if (ids[i] < 0) {
+            ids[i] = 0;
+            reindex_from = i - 1;
+        }
+    }
+    if (reindex_from < n)
+        for (int64_t i = reindex_from; i >= 0; i--) ids[i]++;
+    std::vector<int> sorted(ids);
+    std::sort(sorted.begin(), sorted.end());
+    const int parts = (int)(std::unique(sorted.begin(), sorted.end()) - sorted.begin());
+    if (parts > 0 && (sorted[0] != 0 || sorted[parts - 1] != parts - 1))
+        fail("partition ids are not 0.." + std::to_string(parts - 1) +
+             " after re-indexing (largest " + std::to_string(sorted[parts - 1]) + ")");
+    tv.n_partitions = std::max(parts, 1);
+    delete[] tv.partitionIDs;
+    delete[] tv.n_per_partition;
+    tv.partitionIDs = new int[(size_t)n];
+    tv.n_per_partition = new int[(size_t)tv.n_partitions]();
+    for (int64_t i = 0; i < n; i++) {
+        tv.partitionIDs[i] = ids[i];
+        tv.n_per_partition[ids[i]]++;
+    }
+}
+
+// One partition holding every point (src/vtk_load.cu:155-162).
+void single_partition(TV_Data &tv) {
+    if (tv.partitionIDs) return;
+    tv.n_partitions = 1;
+    tv.partitionIDs = new int[(size_t)std::max<int64_t>(tv.nPoints, 1)]();
+    delete[] tv.n_per_partition;
+    tv.n_per_partition = new int[1];
+    tv.n_per_partition[0] = (int)tv.nPoints;
+}
 
 std::map<std::string, std::string> parse_attrs(const std::string &s) {
     std::map<std::string, std::string> a;
@@ -427,6 +482,55 @@ std::shared_ptr<TV_Data> load_vtu(const runtime_arguments &args) {
         }
         if (!picked && !args.arrayname.empty()) fail("no point array named " + args.arrayname);
     }
+    if (!args.partitioningname.empty()) {
+        bool found = false;
+        if (!pd.self_closing)
+            for (size_t p = pd.body_begin; !found && find_tag(doc, "DataArray", p, pd.body_end, da);
+                 p = da.body_end) {
+                if (!da.attr.count("Name") || da.attr.at("Name") != args.partitioningname) continue;
+                std::vector<double> s = r.as_double(da);
+                const int comps = da.attr.count("NumberOfComponents")
+                                      ? std::atoi(da.attr.at("NumberOfComponents").c_str())
+                                      : 1;
+                if ((int64_t)s.size() != nPoints * comps)
+                    fail("point array " + args.partitioningname + " has the wrong size");
+                std::vector<double> first((size_t)nPoints);
+                for (int64_t v = 0; v < nPoints; v++) first[v] = s[v * comps];
+                assign_partitions(*tv, first);
+                found = true;
+            }
+        if (!found)
+            fail("Unable to find partitioning name '" + args.partitioningname + "' in dataset");
+        std::cout << "Using " << tv->n_partitions << " partitions from " << args.partitioningname
+                  << "\n";
+        // per-partition cell lists stored in the file (cell arrays
+        // partition_cells_<p>, src/vtk_load.cu:261-292); used only when every
+        // partition has one, else the vertex rule applies
+        Tag cd;
+        if (tv->n_partitions > 1 &&
+            find_tag(doc, "CellData", piece.body_begin, piece.body_end, cd) && !cd.self_closing) {
+            std::map<int, std::vector<double>> lists;
+            for (size_t p = cd.body_begin; find_tag(doc, "DataArray", p, cd.body_end, da);
+                 p = da.body_end) {
+                const std::string name = da.attr.count("Name") ? da.attr.at("Name") : "";
+                if (name.compare(0, 16, "partition_cells_") != 0) continue;
+                std::vector<double> s = r.as_double(da);
+                if ((int64_t)s.size() != nCells) fail("cell array " + name + " has the wrong size");
+                lists[std::atoi(name.c_str() + 16)] = std::move(s);
+            }
+            bool all = true;
+            for (int q = 0; q < tv->n_partitions; q++) all = all && lists.count(q);
+            if (all) {
+                tv->partitionCells = new int[(size_t)tv->n_partitions * (size_t)nCells];
+                for (int q = 0; q < tv->n_partitions; q++)
+                    for (int64_t c = 0; c < nCells; c++)
+                        tv->partitionCells[(size_t)q * nCells + c] = (int)lists[q][c];
+            } else if (!lists.empty()) {
+                std::cerr << "Failed to find all partition cells (cellData) arrays, falling back "
+                             "to the vertex rule\n";
+            }
+        }
+    }
     std::cout << "Dataset loaded with " << nCells << " tetrahedra and " << nPoints
               << " vertices\n";
     return tv;
@@ -435,6 +539,8 @@ std::shared_ptr<TV_Data> load_vtu(const runtime_arguments &args) {
 }  // namespace
 
 std::shared_ptr<TV_Data> get_TV_from_VTK(const runtime_arguments args) {
-    if (args.fileName.rfind("kuhn:", 0) == 0) return load_kuhn(args.fileName);
-    return load_vtu(args);
+    std::shared_ptr<TV_Data> tv =
+        args.fileName.rfind("kuhn:", 0) == 0 ? load_kuhn(args.fileName) : load_vtu(args);
+    single_partition(*tv);
+    return tv;
 }

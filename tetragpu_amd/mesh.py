@@ -34,6 +34,34 @@ def init(device: int = 0) -> None:
     check(_native.hip().tg_init(device))
 
 
+def partitioned_classes(cells, n_points: int, scalars, parts):
+    """classes (V, 3) from the reference's partition loop on the device, one
+    partition's cells at a time (src/alg/critPoints.cu:654-1003, merged as at
+    :1438-1491; libtetra_host.so, tetra_partitioned_classes).  Partition p
+    is classified on the cells that hold one of its vertices; only its own
+    vertices keep their classes.  Returns (classes, stats)."""
+    cells = np.ascontiguousarray(cells, dtype=np.int32).reshape(-1, 4)
+    sc = np.ascontiguousarray(scalars, dtype=np.float64)
+    pa = np.ascontiguousarray(parts, dtype=np.int32)
+    if sc.shape[0] != n_points or pa.shape[0] != n_points:
+        raise ValueError("scalars and parts need one entry per point")
+    out = np.zeros((n_points, 3), np.uint32)
+    st = np.zeros(7, np.float64)
+    rc = _native.host().tetra_partitioned_classes(cells.ctypes.data, cells.shape[0], n_points,
+                                                  sc.ctypes.data, pa.ctypes.data,
+                                                  out.ctypes.data, st.ctypes.data)
+    if rc != 0:
+        msg = _native.host().tetra_partition_error().decode(errors="replace")
+        if rc == -1:
+            raise ValueError(msg)
+        raise _native.NativeError(rc, msg)
+    keys = ("partitions", "total_s", "localize_s", "classify_s", "max_cells", "max_points",
+            "cells_total")
+    stats = {k: (int(v) if k in ("partitions", "max_cells", "max_points", "cells_total")
+                 else float(v)) for k, v in zip(keys, st)}
+    return out, stats
+
+
 def device_count() -> int:
     n = C.c_int(0)
     _native.hip().tg_device_count(C.byref(n))
