@@ -377,6 +377,15 @@ __global__ __launch_bounds__(256) void k_rows_compact(
     }
     __syncthreads();
     const int32_t ob = s_off[0], oe = s_off[nv];
+    // rows that sit back to back in scratch (a bucket of 256 vertices with
+    // no heavy vertex) move as one run
+    const int32_t shift = s_src[0] - ob;
+    bool run = true;
+    for (int i = threadIdx.x; i < nv; i += 256) run = run && s_src[i] - s_off[i] == shift;
+    if (__syncthreads_and(run)) {
+        for (int32_t o = ob + threadIdx.x; o < oe; o += 256) cols[o] = scratch[(int64_t)o + shift];
+        return;
+    }
     for (int32_t o = ob + threadIdx.x; o < oe; o += 256) {
         int lo = 0, hi = nv - 1;
         while (lo < hi) {
