@@ -151,3 +151,29 @@ def test_relations_after_hot_path(gpu):
         same_csr(d.relation("ET"), ref["ET"], "ET")
         same_csr(d.relation("TT"), ref["TT"], "TT")
         same_csr(d.relation("VF"), ref["VF"], "VF")
+
+
+@pytest.mark.parametrize("bv", [7, 8])
+@pytest.mark.parametrize("route", [1, 2])
+@pytest.mark.parametrize("case", ["dense", "kuhn", "partial"])
+def test_vt_bucketed_routes(gpu, case, route, bv):
+    # VT from the routed corner records: both routings, both bucket sizes,
+    # buckets over the LDS capacity (rows staged in global memory) and a
+    # partial last bucket with unreferenced vertices
+    if case == "dense":
+        rng = np.random.default_rng(31)
+        V = 200
+        cells = np.array([rng.choice(V, 4, replace=False) for _ in range(10000)], np.int32)
+    elif case == "kuhn":
+        m = gpu.kuhn(9, seed=4)
+        cells, V = m.cells, m.n_points
+    else:
+        m = gpu.kuhn(5, seed=2)
+        cells, V = m.cells, m.n_points + 77
+    print(f"vt case={case} route={route} bv={bv}", flush=True)
+    with gpu.Mesh(cells, V, None) as d:
+        d.set_bucket(bv, 4096)
+        d.set_routing(route)
+        d.build(gpu.flags("VT"))
+        same_csr(d.relation("VT"), O.vt(cells, V), "VT")
+        assert d.approx_max_vv() == O.approx_max_vv(cells, V)

@@ -44,5 +44,11 @@ BucketPlan plan_buckets(int64_t n_cells, int64_t n_points, int bv_log = 7, int64
 int launch_bucketed_vv_cp(BucketBuffers &k, bool classify, hipStream_t s);
 // Whole bucketed pipeline: counters, routing, VV rows, CSR, link pass.
 int run_vv_cp_bucketed(BucketBuffers &k, bool classify, hipStream_t s);
+// VT corner lists (codes 4t + j, rows ascending) through the same routing:
+// vt_off [V + 1], vtc [4T], vt_cnt [V] (row lengths).  rec1 and rec2 are
+// scratch here: they must not alias live data (the mesh passes its own
+// buffers, not the VV columns).  Needs 4T <= 2^31 - 1.
+int run_vt_bucketed(BucketBuffers &k, int32_t *vt_off, int32_t *vtc, int32_t *vt_cnt,
+                    hipStream_t s);
 
 }  // namespace tg

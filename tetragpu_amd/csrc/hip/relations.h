@@ -43,6 +43,7 @@ struct tg_mesh {
     tg::DevArr vt_cnt, vt_off, vt_cur, vtc, scratch, deg, vv_off, vv_cols, classes,
         heavy, fix, fixbig, counters, scan_tmp, labels;
     tg::DevArr bk_h1, bk_h2, bk_rowsrc, bk_fixd, bk_rowdesc;
+    tg::DevArr bk_rec;  // level-2 records of the bucketed VT build (released after it)
     tg::BucketPlan bplan;
     int32_t bk_route = 0;  // tg::Route of the bucketed pipeline (AUTO until the first run)
     // measured at creation: tiles of 1024 cells, and those whose vertices span

@@ -54,7 +54,7 @@ static void release_all_impl(tg_mesh *m, hipStream_t s, bool release) {
                          &me.fe,       &me.ft,       &me.ef_off,   &me.ef_vals,  &me.tt_off,   &me.tt_vals,
                          &me.ff_off,   &me.ff_vals,  &me.ee_off,   &me.ee_vals,  &me.bitmap,   &me.rank_tmp,
                          &me.tmp_a,    &me.tmp_b,    &me.tmp_c,    &me.bk_h1,    &me.bk_h2,    &me.bk_rowsrc,
-                         &me.bk_fixd,  &me.bk_rowdesc};
+                         &me.bk_fixd,  &me.bk_rowdesc, &me.bk_rec};
     for (auto *a : all) {
         a->stream = s;
         if (release) a->release();
@@ -238,10 +238,39 @@ static int run_vv(tg_mesh &m, bool classify) {
 
 int run_hot_path(tg_mesh &m) { return run_vv(m, true); }
 
+// VT through the bucketed routing when the mesh fits it: rows come out
+// ascending, with no global atomics and no row sort.  Level-1 records use
+// the hot path's row scratch (dead between runs), level-2 records a buffer
+// of their own that is released afterwards.
+static int vt_bucketed(tg_mesh &m, bool &done) {
+    done = false;
+    int rc = prepare_hot_path(m);
+    if (rc) return rc;
+    if (!m.bplan.ok || 4 * m.n_cells > (int64_t)INT32_MAX) return OK;
+    if ((rc = prepare_buckets(m))) return rc;
+    if ((rc = m.bk_rec.ensure(12 * sizeof(int32_t) * (size_t)std::max<int64_t>(m.n_cells, 1))))
+        return rc;
+    BucketBuffers k = bucket_buffers(m);
+    k.rec2 = m.bk_rec.as<uint32_t>();
+    if ((rc = run_vt_bucketed(k, m.vt_off.as<int32_t>(), m.vtc.as<int32_t>(),
+                              m.vt_cnt.as<int32_t>(), m.stream)))
+        return rc;
+    m.bk_rec.release();
+    done = true;
+    return OK;
+}
+
 int ensure_vt_corners(tg_mesh &m) {
     if (m.vt_corners_built) return OK;
     int rc = prepare_vt(m);
     if (rc) return rc;
+    bool done = false;
+    if ((rc = vt_bucketed(m, done))) return rc;
+    if (done) {
+        m.vt_corners_built = true;
+        m.vt_sorted = true;
+        return OK;
+    }
     VvCpBuffers b = hot_buffers(m);
     if ((rc = launch_vt_corners(b, m.cfg.corner_block, m.stream))) return rc;
     m.vt_corners_built = true;

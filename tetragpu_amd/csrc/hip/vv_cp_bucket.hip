@@ -163,6 +163,7 @@ __device__ __forceinline__ void stage_flush(ST &st, Rec out, int nthreads, int s
     }
 }
 
+template <bool VTREC>
 __global__ __launch_bounds__(L1_THREADS) void k_l1_scatter(const int4 *__restrict__ tv4, int64_t T,
                                                             int64_t tpc, int cshift, int32_t nc,
                                                             int32_t nch,
@@ -210,10 +211,18 @@ __global__ __launch_bounds__(L1_THREADS) void k_l1_scatter(const int4 *__restric
                 st.w1[slot] = (a >> 8) | (b << 16);
                 st.w2[slot] = (b >> 16) | (c << 8);
             };
-            put(q[k].x >> cshift, r[k][0], x, y, z, w);
-            put(q[k].y >> cshift, r[k][1], y, x, z, w);
-            put(q[k].z >> cshift, r[k][2], z, x, y, w);
-            put(q[k].w >> cshift, r[k][3], w, x, y, z);
+            if constexpr (VTREC) {
+                const uint32_t c0 = 4u * (uint32_t)(tb + k * L1_THREADS + tid);
+                put(q[k].x >> cshift, r[k][0], x, c0 & M24, c0 >> 24, 0u);
+                put(q[k].y >> cshift, r[k][1], y, (c0 + 1) & M24, (c0 + 1) >> 24, 0u);
+                put(q[k].z >> cshift, r[k][2], z, (c0 + 2) & M24, (c0 + 2) >> 24, 0u);
+                put(q[k].w >> cshift, r[k][3], w, (c0 + 3) & M24, (c0 + 3) >> 24, 0u);
+            } else {
+                put(q[k].x >> cshift, r[k][0], x, y, z, w);
+                put(q[k].y >> cshift, r[k][1], y, x, z, w);
+                put(q[k].z >> cshift, r[k][2], z, x, y, w);
+                put(q[k].w >> cshift, r[k][3], w, x, y, z);
+            }
         }
         __syncthreads();
         // next tile's tets are in flight during the flush
@@ -425,6 +434,7 @@ struct S1Stage {
 };
 
 // bstart: exclusive scan of the bucket counts; bused: zeroed fill counters.
+template <bool VTREC>
 __global__ __launch_bounds__(S1_THREADS) void k_s1_scatter(const int4 *__restrict__ tv4,
                                                             int64_t T, int64_t tpc, int bv_log,
                                                             const int32_t *__restrict__ bstart,
@@ -478,10 +488,18 @@ __global__ __launch_bounds__(S1_THREADS) void k_s1_scatter(const int4 *__restric
                     st.w2[slot] = (b >> 16) | (cc << 8);
                     st.bin[slot] = (uint16_t)j;
                 };
-                put(b0 - bmin, r0, x, y, z, w);
-                put(b1 - bmin, r1, y, x, z, w);
-                put(b2 - bmin, r2, z, x, y, w);
-                put(b3 - bmin, r3, w, x, y, z);
+                if constexpr (VTREC) {
+                    const uint32_t c0 = 4u * (uint32_t)(tb + tid);
+                    put(b0 - bmin, r0, x, c0 & M24, c0 >> 24, 0u);
+                    put(b1 - bmin, r1, y, (c0 + 1) & M24, (c0 + 1) >> 24, 0u);
+                    put(b2 - bmin, r2, z, (c0 + 2) & M24, (c0 + 2) >> 24, 0u);
+                    put(b3 - bmin, r3, w, (c0 + 3) & M24, (c0 + 3) >> 24, 0u);
+                } else {
+                    put(b0 - bmin, r0, x, y, z, w);
+                    put(b1 - bmin, r1, y, x, z, w);
+                    put(b2 - bmin, r2, z, x, y, w);
+                    put(b3 - bmin, r3, w, x, y, z);
+                }
             }
             __syncthreads();
             for (int slot = tid; slot < total; slot += S1_THREADS) {
@@ -492,10 +510,21 @@ __global__ __launch_bounds__(S1_THREADS) void k_s1_scatter(const int4 *__restric
                 out.w2[pos] = st.w2[slot];
             }
         } else if (on) {
-            rec_store(out, (int64_t)bstart[b0] + atomicAdd(&bused[b0], 1), x, y, z, w);
-            rec_store(out, (int64_t)bstart[b1] + atomicAdd(&bused[b1], 1), y, x, z, w);
-            rec_store(out, (int64_t)bstart[b2] + atomicAdd(&bused[b2], 1), z, x, y, w);
-            rec_store(out, (int64_t)bstart[b3] + atomicAdd(&bused[b3], 1), w, x, y, z);
+            if constexpr (VTREC) {
+                const uint32_t c0 = 4u * (uint32_t)(tb + tid);
+                rec_store(out, (int64_t)bstart[b0] + atomicAdd(&bused[b0], 1), x, c0 & M24, c0 >> 24, 0u);
+                rec_store(out, (int64_t)bstart[b1] + atomicAdd(&bused[b1], 1), y, (c0 + 1) & M24,
+                          (c0 + 1) >> 24, 0u);
+                rec_store(out, (int64_t)bstart[b2] + atomicAdd(&bused[b2], 1), z, (c0 + 2) & M24,
+                          (c0 + 2) >> 24, 0u);
+                rec_store(out, (int64_t)bstart[b3] + atomicAdd(&bused[b3], 1), w, (c0 + 3) & M24,
+                          (c0 + 3) >> 24, 0u);
+            } else {
+                rec_store(out, (int64_t)bstart[b0] + atomicAdd(&bused[b0], 1), x, y, z, w);
+                rec_store(out, (int64_t)bstart[b1] + atomicAdd(&bused[b1], 1), y, x, z, w);
+                rec_store(out, (int64_t)bstart[b2] + atomicAdd(&bused[b2], 1), z, x, y, w);
+                rec_store(out, (int64_t)bstart[b3] + atomicAdd(&bused[b3], 1), w, x, y, z);
+            }
         }
         __syncthreads();  // the stage is reused by the next tile
     }
@@ -1255,14 +1284,91 @@ __global__ __launch_bounds__(HV_THREADS) void k_bkt_heavy(
     }
 }
 
-template <typename S>
-int launch_bucketed_typed(tg::BucketBuffers &k, bool classify, hipStream_t s) {
+// ---- VT rows from the routed corner records -------------------------------------
+// One workgroup per bucket.  The bucket's records (v, 4t + j) are counted per
+// vertex, placed per row (LDS, or the dead level-1 array `stage` for buckets
+// with more than VT_CAP records), and every entry then takes its place in
+// its ascending row by counting the smaller codes of that row (codes in a
+// row are distinct).  The bucket's rows fill the same range of the CSR that
+// its records fill in bucket order, so row offsets need no global scan.
+constexpr int VT_THREADS = 256, VT_CAP = 8192;
+
+template <int BV_LOG>
+__global__ __launch_bounds__(VT_THREADS) void k_bkt_vt(const Rec rec,
+                                                        const int32_t *__restrict__ h2s,
+                                                        int32_t nslice, int32_t V, int32_t nb,
+                                                        int32_t *__restrict__ stage,
+                                                        int32_t *__restrict__ vt_off,
+                                                        int32_t *__restrict__ vtc,
+                                                        int32_t *__restrict__ vt_cnt) {
+    constexpr int BV = 1 << BV_LOG;
+    static_assert(BV <= VT_THREADS, "one thread per vertex in the scan");
+    __shared__ int32_t cnt[BV], off[BV + 1], cur[BV];
+    __shared__ int32_t buf[VT_CAP];
+    __shared__ int32_t wsum[VT_THREADS / 64];
+    const int tid = threadIdx.x, lane = __lane_id(), wid = tid / 64;
+    const int32_t b = blockIdx.x, v0 = b << BV_LOG;
+    const int nv = min(BV, V - v0);
+    const int32_t cbeg = h2s[(int64_t)b * nslice], cend = h2s[(int64_t)(b + 1) * nslice];
+    const int32_t n = cend - cbeg;
+    int32_t *st = n <= VT_CAP ? buf : stage + cbeg;
+    for (int i = tid; i < BV; i += VT_THREADS) cnt[i] = 0;
+    __syncthreads();
+    for (int32_t i = cbeg + tid; i < cend; i += VT_THREADS)
+        atomicAdd(&cnt[rec.w0[i] & (uint32_t)(BV - 1)], 1);
+    __syncthreads();
+    const int32_t x = tid < BV ? cnt[tid] : 0;
+    const int32_t inc = tgd::wave_inclusive_sum(x);
+    if (lane == 63) wsum[wid] = inc;
+    __syncthreads();
+    int32_t pre = 0;
+#pragma unroll
+    for (int w = 0; w < VT_THREADS / 64; w++)
+        if (w < wid) pre += wsum[w];
+    if (tid < BV) {
+        off[tid] = pre + inc - x;
+        cur[tid] = pre + inc - x;
+    }
+    if (tid == 0) off[BV] = n;
+    __syncthreads();
+    if (tid < nv) {
+        vt_off[v0 + tid] = cbeg + off[tid];
+        vt_cnt[v0 + tid] = x;
+    }
+    if (b == nb - 1 && tid == 0) vt_off[V] = cend;
+    for (int32_t i = cbeg + tid; i < cend; i += VT_THREADS) {
+        uint32_t v, lo, hi, unused;
+        rec_unpack(rec.w0[i], rec.w1[i], rec.w2[i], v, lo, hi, unused);
+        const int32_t pos = atomicAdd(&cur[v & (uint32_t)(BV - 1)], 1);
+        st[pos] = (int32_t)(lo | (hi << 24));
+    }
+    __threadfence();
+    __syncthreads();
+    for (int32_t p = tid; p < n; p += VT_THREADS) {
+        int lo = 0, hi = BV - 1;  // row of entry p: last vertex with off <= p
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (off[mid] <= p) lo = mid;
+            else hi = mid - 1;
+        }
+        const int32_t rb = off[lo], re = off[lo + 1], e = st[p];
+        int32_t r = 0;
+        for (int32_t q = rb; q < re; q++) r += st[q] < e;
+        vtc[cbeg + rb + r] = e;
+    }
+}
+
+// Route the corner records of every cell to the buckets of their vertex:
+// r2 holds them in bucket order, bucket b in [ranges[b * nsl], ranges[(b + 1)
+// * nsl]).  One level (ranges = h2[0..nb]) or two levels (h2 is
+// [bucket][slice]); the choice is made on the mesh's first run.  VTREC:
+// records carry the corner code 4t + j (low 24 bits, high bits) instead of
+// the other three corners.
+template <bool VTREC>
+int route_records(tg::BucketBuffers &k, hipStream_t s, const int32_t *&ranges, int32_t &nsl) {
     const tg::BucketPlan &p = k.plan;
     tg::VvCpBuffers &b = k.base;
-    const int32_t V = (int32_t)b.n_points;
     const int64_t T = b.n_cells;
-    const int cls = classify ? 1 : 0;
-    const S *scal = (const S *)b.scalars;
     const int4 *tv4 = (const int4 *)b.tv;
     tg::StageMarks *mk = b.marks;
     const int64_t n = 4 * T;
@@ -1270,12 +1376,9 @@ int launch_bucketed_typed(tg::BucketBuffers &k, bool classify, hipStream_t s) {
     const Rec r2{k.rec2, k.rec2 + n, k.rec2 + 2 * n};
     const int cshift = p.bv_log + p.fs;
     int rc = tg::OK;
-
-    // routing: one level (bucket ranges in h2[0..nb]) or two levels (h2 is
-    // [bucket][slice]); the choice is made on the mesh's first run
     int route = k.route ? *k.route : tg::ROUTE_TWO;
-    const int32_t *ranges = k.h2;
-    int32_t nsl = p.slices;
+    ranges = k.h2;
+    nsl = p.slices;
     if (route != tg::ROUTE_TWO) {
         int32_t *bstart = k.h2, *bused = k.h1;
         if ((rc = tg::fill_i32(bstart, 0, p.nb, s)) || (rc = tg::fill_i32(bused, 0, p.nb, s)))
@@ -1303,7 +1406,7 @@ int launch_bucketed_typed(tg::BucketBuffers &k, bool classify, hipStream_t s) {
         if (route == tg::ROUTE_ONE) {
             if ((rc = tg::scan_exclusive_i32(bstart, bstart, p.nb, b.scan_tmp, s))) return rc;
             if (mk) mk->mark("s1_count", s);
-            hipLaunchKernelGGL(k_s1_scatter, dim3(p.nchunks), dim3(S1_THREADS), 0, s, tv4, T,
+            hipLaunchKernelGGL(k_s1_scatter<VTREC>, dim3(p.nchunks), dim3(S1_THREADS), 0, s, tv4, T,
                                p.tpc, p.bv_log, bstart, bused, r2);
             TG_LAUNCH_CHECK("k_s1_scatter");
             if (mk) mk->mark("s1_scatter", s);
@@ -1318,7 +1421,7 @@ int launch_bucketed_typed(tg::BucketBuffers &k, bool classify, hipStream_t s) {
         if ((rc = tg::scan_exclusive_i32(k.h1, k.h1, (int64_t)p.nc * p.nchunks, b.scan_tmp, s)))
             return rc;
         if (mk) mk->mark("l1_count", s);
-        hipLaunchKernelGGL(k_l1_scatter, dim3(p.nchunks), dim3(L1_THREADS), 0, s, tv4, T, p.tpc,
+        hipLaunchKernelGGL(k_l1_scatter<VTREC>, dim3(p.nchunks), dim3(L1_THREADS), 0, s, tv4, T, p.tpc,
                            cshift, p.nc, p.nchunks, k.h1, r1);
         TG_LAUNCH_CHECK("k_l1_scatter");
         if (mk) mk->mark("l1_scatter", s);
@@ -1334,6 +1437,24 @@ int launch_bucketed_typed(tg::BucketBuffers &k, bool classify, hipStream_t s) {
         TG_LAUNCH_CHECK("k_l2_scatter");
         if (mk) mk->mark("l2_scatter", s);
     }
+    return tg::OK;
+}
+
+template <typename S>
+int launch_bucketed_typed(tg::BucketBuffers &k, bool classify, hipStream_t s) {
+    const tg::BucketPlan &p = k.plan;
+    tg::VvCpBuffers &b = k.base;
+    const int32_t V = (int32_t)b.n_points;
+    const int64_t T = b.n_cells;
+    const int cls = classify ? 1 : 0;
+    const S *scal = (const S *)b.scalars;
+    tg::StageMarks *mk = b.marks;
+    const int64_t n = 4 * T;
+    const Rec r2{k.rec2, k.rec2 + n, k.rec2 + 2 * n};
+    const int32_t *ranges = nullptr;
+    int32_t nsl = 1;
+    int rc = route_records<false>(k, s, ranges, nsl);
+    if (rc) return rc;
 #define TG_BKT(KERNEL, BVL)                                                                  \
     hipLaunchKernelGGL((KERNEL<BVL, S>), dim3(p.nb), dim3(2 << BVL), 0, s, r2, ranges, nsl,   \
                        scal, V, cls, b.scratch, b.deg, k.rowsrc, b.classes, b.heavy, b.fixd,   \
@@ -1390,6 +1511,31 @@ BucketPlan plan_buckets(int64_t n_cells, int64_t n_points, int bv_log, int64_t t
 int launch_bucketed_vv_cp(BucketBuffers &k, bool classify, hipStream_t s) {
     return k.base.scalars_f64 ? launch_bucketed_typed<double>(k, classify, s)
                               : launch_bucketed_typed<float>(k, classify, s);
+}
+
+int run_vt_bucketed(BucketBuffers &k, int32_t *vt_off, int32_t *vtc, int32_t *vt_cnt,
+                    hipStream_t s) {
+    const BucketPlan &p = k.plan;
+    VvCpBuffers &b = k.base;
+    const int32_t V = (int32_t)b.n_points;
+    if (V <= 0) return fill_i32(vt_off, 0, 1, s);
+    if (4 * b.n_cells > (int64_t)INT32_MAX) return ERR_CAPACITY;
+    int rc = fill_i32(b.counters + C_WIDE, 0, 1, s);  // the routing choice counts wide tiles
+    if (rc) return rc;
+    const int32_t *ranges = nullptr;
+    int32_t nsl = 1;
+    if ((rc = route_records<true>(k, s, ranges, nsl))) return rc;
+    const int64_t n = 4 * b.n_cells;
+    const Rec r2{k.rec2, k.rec2 + n, k.rec2 + 2 * n};
+    int32_t *stage = (int32_t *)k.rec1;
+    if (p.bv_log == 7)
+        hipLaunchKernelGGL((k_bkt_vt<7>), dim3(p.nb), dim3(VT_THREADS), 0, s, r2, ranges, nsl, V,
+                           p.nb, stage, vt_off, vtc, vt_cnt);
+    else
+        hipLaunchKernelGGL((k_bkt_vt<8>), dim3(p.nb), dim3(VT_THREADS), 0, s, r2, ranges, nsl, V,
+                           p.nb, stage, vt_off, vtc, vt_cnt);
+    TG_LAUNCH_CHECK("k_bkt_vt");
+    return OK;
 }
 
 }  // namespace tg
