@@ -298,6 +298,8 @@ __global__ __launch_bounds__(ET_THREADS) void k_et(const int32_t *__restrict__ e
 // positional triple) and counts all matching tets.
 // Ids read from TE and ET are checked against E and T: an inconsistent
 // relation yields INF (reported by the callers) instead of a stray access.
+constexpr int FF_ROW = 8;  // ET rows up to this length take the register path
+
 __device__ __forceinline__ int32_t face_first(const int4 *tv4, int64_t T, const int32_t *te,
                                               int64_t E, const int64_t *et_off,
                                               const int32_t *et, int32_t t, int s,
@@ -312,6 +314,46 @@ __device__ __forceinline__ int32_t face_first(const int4 *tv4, int64_t T, const 
     nmatch = 0;
     if (e1 < 0 || e1 >= E || e2 < 0 || e2 >= E) return best;
     int64_t i = et_off[e1], ie = et_off[e1 + 1], j = et_off[e2], je = et_off[e2 + 1];
+    if (ie - i <= FF_ROW && je - j <= FF_ROW) {
+        // short rows: both are loaded at once and intersected in registers,
+        // and the cells they share are gathered together, so the slot costs
+        // two dependent loads instead of one per row element
+        const int na = (int)(ie - i), nb = (int)(je - j);
+        int32_t ra[FF_ROW], rb[FF_ROW];
+#pragma unroll
+        for (int k = 0; k < FF_ROW; k++) {
+            ra[k] = k < na ? et[i + k] : 0;
+            rb[k] = k < nb ? et[j + k] : 0;
+        }
+        bool hit[FF_ROW];
+        int4 r[FF_ROW];
+#pragma unroll
+        for (int k = 0; k < FF_ROW; k++) {
+            bool h = false;
+#pragma unroll
+            for (int l = 0; l < FF_ROW; l++) h = h || (l < nb && rb[l] == ra[k]);
+            hit[k] = k < na && h;
+            if (hit[k] && (ra[k] < 0 || ra[k] >= T)) return INF;
+        }
+#pragma unroll
+        for (int k = 0; k < FF_ROW; k++) r[k] = hit[k] ? tv4[ra[k]] : make_int4(-1, -1, -1, -1);
+#pragma unroll
+        for (int k = 0; k < FF_ROW; k++) {
+            if (!hit[k]) continue;
+            for (int s2 = 0; s2 < 4; s2++) {
+                int l2, m2, h2;
+                face_pos(s2, l2, m2, h2);
+                if (corner(r[k], l2) == a && corner(r[k], m2) == b && corner(r[k], h2) == c) {
+                    if (tets2 && nmatch == 0) tets2[0] = ra[k];
+                    if (tets2 && nmatch == 1) tets2[1] = ra[k];
+                    nmatch++;
+                    best = min(best, 4 * ra[k] + s2);
+                    break;
+                }
+            }
+        }
+        return best;
+    }
     while (i < ie && j < je) {
         const int32_t x = et[i], y = et[j];
         if (x < y) i++;
