@@ -72,17 +72,66 @@ __device__ __forceinline__ int32_t bsearch_i32(const int32_t *a, int32_t n, int3
 // Every pair of v's cells whose low vertex is v also records the VV index of
 // its edge in te (replaced by the edge id once ids exist): the row search is
 // done here, on v's own row, instead of once per cell on random rows.
-__global__ void k_edge_first(const int4 *__restrict__ tv4, const int32_t *__restrict__ vt_off,
-                             const int32_t *__restrict__ vtc, const int32_t *__restrict__ vv_off,
-                             const int32_t *__restrict__ vv, int64_t V,
-                             int32_t *__restrict__ efirst, uint8_t *__restrict__ flags,
-                             int32_t *__restrict__ te_idx) {
-    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+// Rows up to EDGE_CAP entries are staged in LDS (one column per thread: the
+// VV row and the first positions), and the vertex's cells are fetched eight
+// at a time with independent loads; longer rows work in global memory.
+constexpr int EDGE_THREADS = 64, EDGE_CAP = 48;
+
+__global__ __launch_bounds__(EDGE_THREADS) void k_edge_first(
+    const int4 *__restrict__ tv4, const int32_t *__restrict__ vt_off,
+    const int32_t *__restrict__ vtc, const int32_t *__restrict__ vv_off,
+    const int32_t *__restrict__ vv, int64_t V, int32_t *__restrict__ efirst,
+    uint8_t *__restrict__ flags, int32_t *__restrict__ te_idx) {
+    __shared__ int32_t srow[EDGE_CAP][EDGE_THREADS];
+    __shared__ int32_t sef[EDGE_CAP][EDGE_THREADS];
+    const int tid = threadIdx.x;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + tid; v < V;
          v += (int64_t)gridDim.x * blockDim.x) {
         const int32_t rb = vv_off[v], rn = vv_off[v + 1] - rb;
         const int32_t *row = vv + rb;
         int32_t *ef = efirst + rb;
-        for (int32_t k = vt_off[v]; k < vt_off[v + 1]; k++) {
+        const int32_t kb = vt_off[v], ke = vt_off[v + 1];
+        if (rn <= EDGE_CAP) {
+#pragma unroll 8
+            for (int32_t i = 0; i < rn; i++) {
+                srow[i][tid] = row[i];
+                sef[i][tid] = INF;
+            }
+            for (int32_t k0 = kb; k0 < ke; k0 += 8) {
+                int32_t pc[8];
+                int4 q[8];
+#pragma unroll
+                for (int c = 0; c < 8; c++) pc[c] = k0 + c < ke ? vtc[k0 + c] : -1;
+#pragma unroll
+                for (int c = 0; c < 8; c++) q[c] = pc[c] >= 0 ? tv4[pc[c] >> 2] : make_int4(0, 0, 0, 0);
+#pragma unroll
+                for (int c = 0; c < 8; c++) {
+                    if (pc[c] < 0) continue;
+                    const int32_t t = pc[c] >> 2, jv = pc[c] & 3;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        if (j == jv) continue;
+                        const int32_t u = corner(q[c], j);
+                        if (u < v) continue;
+                        int32_t lo = 0;  // lower bound of u in the staged row (rn < 64)
+#pragma unroll
+                        for (int step = 32; step >= 1; step >>= 1)
+                            if (lo + step - 1 < rn && srow[lo + step - 1][tid] < u) lo += step;
+                        const int32_t idx = lo < rn && srow[lo][tid] == u ? lo : -1;
+                        const int32_t pos = 6 * t + pair_index(min(j, (int)jv), max(j, (int)jv));
+                        te_idx[pos] = idx >= 0 ? rb + idx : -1;
+                        if (idx >= 0 && sef[idx][tid] == INF) sef[idx][tid] = pos;
+                    }
+                }
+            }
+            for (int32_t i = 0; i < rn; i++) {
+                const int32_t e = sef[i][tid];
+                ef[i] = e;
+                if (srow[i][tid] > v && e != INF) flags[e] = 1;
+            }
+            continue;
+        }
+        for (int32_t k = kb; k < ke; k++) {
             const int32_t p = vtc[k];
             const int32_t t = p >> 2, jv = p & 3;
             const int4 q = tv4[t];
@@ -862,7 +911,7 @@ int ensure_edges(tg_mesh &m) {
                        m.edge_first.as<int32_t>(), INF, (int64_t)nnz);
     TG_LAUNCH_CHECK("k_fill_i32v");
     if (V > 0) {
-        hipLaunchKernelGGL(k_edge_first, dim3(grid(V)), dim3(256), 0, m.stream,
+        hipLaunchKernelGGL(k_edge_first, dim3(grid(V, EDGE_THREADS)), dim3(EDGE_THREADS), 0, m.stream,
                            m.tv.as<const int4>(), m.vt_off.as<int32_t>(), m.vtc.as<int32_t>(),
                            m.vv_off.as<int32_t>(), m.vv_cols.as<int32_t>(), V,
                            m.edge_first.as<int32_t>(), flags, m.te.as<int32_t>());
