@@ -1293,6 +1293,54 @@ __global__ __launch_bounds__(HV_THREADS) void k_bkt_heavy(
 // its records fill in bucket order, so row offsets need no global scan.
 constexpr int VT_THREADS = 256, VT_CAP = 8192;
 
+template <int BV, typename P>
+__device__ __forceinline__ void vt_place_rank(const Rec &rec, int32_t cbeg, int32_t cend,
+                                              int32_t *cur, const int32_t *off, P st,
+                                              int32_t *__restrict__ vtc) {
+    const int tid = threadIdx.x;
+    for (int32_t i = cbeg + tid; i < cend; i += VT_THREADS) {
+        uint32_t v, lo, hi, unused;
+        rec_unpack(rec.w0[i], rec.w1[i], rec.w2[i], v, lo, hi, unused);
+        const int32_t pos = atomicAdd(&cur[v & (uint32_t)(BV - 1)], 1);
+        st[pos] = (int32_t)(lo | (hi << 24));
+    }
+    // the workgroup barrier orders the staged writes for its own reads, LDS
+    // or global; a device-scope fence here would write back L2 per block
+    __syncthreads();
+    // rows of at most 32 entries: one thread per vertex sorts its row in
+    // registers; longer rows are listed and ranked by the whole block
+    __shared__ int32_t nlong;
+    __shared__ int32_t lng[BV];
+    if (tid == 0) nlong = 0;
+    __syncthreads();
+    if (tid < BV) {
+        const int32_t rb = off[tid], d = off[tid + 1] - rb;
+        if (d > 32) {
+            lng[atomicAdd(&nlong, 1)] = tid;
+        } else if (d > 0) {
+            uint32_t r[32];
+#pragma unroll
+            for (int i = 0; i < 32; i++) r[i] = i < d ? (uint32_t)st[rb + i] : 0xFFFFFFFFu;
+            sort32(r);
+#pragma unroll
+            for (int i = 0; i < 32; i++)
+                if (i < d) vtc[cbeg + rb + i] = (int32_t)r[i];
+        }
+    }
+    __syncthreads();
+    for (int k = 0; k < nlong; k++) {
+        const int vl = lng[k];
+        const int32_t rb = off[vl], re = off[vl + 1];
+        for (int32_t p = rb + tid; p < re; p += VT_THREADS) {
+            const int32_t e = st[p];
+            int32_t r = 0;
+#pragma unroll 4
+            for (int32_t q = rb; q < re; q++) r += st[q] < e;
+            vtc[cbeg + rb + r] = e;
+        }
+    }
+}
+
 template <int BV_LOG>
 __global__ __launch_bounds__(VT_THREADS) void k_bkt_vt(const Rec rec,
                                                         const int32_t *__restrict__ h2s,
@@ -1311,7 +1359,6 @@ __global__ __launch_bounds__(VT_THREADS) void k_bkt_vt(const Rec rec,
     const int nv = min(BV, V - v0);
     const int32_t cbeg = h2s[(int64_t)b * nslice], cend = h2s[(int64_t)(b + 1) * nslice];
     const int32_t n = cend - cbeg;
-    int32_t *st = n <= VT_CAP ? buf : stage + cbeg;
     for (int i = tid; i < BV; i += VT_THREADS) cnt[i] = 0;
     __syncthreads();
     for (int32_t i = cbeg + tid; i < cend; i += VT_THREADS)
@@ -1336,26 +1383,10 @@ __global__ __launch_bounds__(VT_THREADS) void k_bkt_vt(const Rec rec,
         vt_cnt[v0 + tid] = x;
     }
     if (b == nb - 1 && tid == 0) vt_off[V] = cend;
-    for (int32_t i = cbeg + tid; i < cend; i += VT_THREADS) {
-        uint32_t v, lo, hi, unused;
-        rec_unpack(rec.w0[i], rec.w1[i], rec.w2[i], v, lo, hi, unused);
-        const int32_t pos = atomicAdd(&cur[v & (uint32_t)(BV - 1)], 1);
-        st[pos] = (int32_t)(lo | (hi << 24));
-    }
-    __threadfence();
-    __syncthreads();
-    for (int32_t p = tid; p < n; p += VT_THREADS) {
-        int lo = 0, hi = BV - 1;  // row of entry p: last vertex with off <= p
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (off[mid] <= p) lo = mid;
-            else hi = mid - 1;
-        }
-        const int32_t rb = off[lo], re = off[lo + 1], e = st[p];
-        int32_t r = 0;
-        for (int32_t q = rb; q < re; q++) r += st[q] < e;
-        vtc[cbeg + rb + r] = e;
-    }
+    // LDS staging when the bucket fits, else the dead level-1 array (a
+    // separate instantiation keeps the LDS accesses on LDS instructions)
+    if (n <= VT_CAP) vt_place_rank<BV>(rec, cbeg, cend, cur, off, buf, vtc);
+    else vt_place_rank<BV>(rec, cbeg, cend, cur, off, stage + cbeg, vtc);
 }
 
 // Route the corner records of every cell to the buckets of their vertex:
