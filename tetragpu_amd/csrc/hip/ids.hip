@@ -522,22 +522,44 @@ __global__ void k_vf(const int4 *__restrict__ tv4, const int32_t *__restrict__ v
          a += (int64_t)gridDim.x * blockDim.x) {
         int32_t *row = FILL ? out + 3 * (int64_t)cnt_or_off[a] : nullptr;
         int32_t n = 0;
-        for (int32_t k = vt_off[a]; k < vt_off[a + 1]; k++) {
-            const int32_t p = vtc[k];
-            const int32_t t = p >> 2, j = p & 3;
-            if (j > 1) continue;  // position 0 is low in slots 0-2, position 1 in slot 3
-            const int4 q = tv4[t];
-            for (int s = (j == 0 ? 0 : 3); s <= (j == 0 ? 2 : 3); s++) {
-                const int64_t pos = 4 * (int64_t)t + s;
-                if (ffirst[pos] != (int32_t)pos) continue;
-                if (FILL) {
-                    int lo, mid, hi;
-                    face_pos(s, lo, mid, hi);
-                    row[3 * n] = corner(q, mid);
-                    row[3 * n + 1] = corner(q, hi);
-                    row[3 * n + 2] = tf[pos];
+        const int32_t kb = vt_off[a], ke = vt_off[a + 1];
+        // eight cells at a time: their slot flags, ids and corners are
+        // loaded together, then the row is written in cell order
+        for (int32_t k0 = kb; k0 < ke; k0 += 8) {
+            int32_t pc[8], ff[8][3], id[8][3];
+            int4 q[8];
+#pragma unroll
+            for (int c = 0; c < 8; c++) pc[c] = k0 + c < ke ? vtc[k0 + c] : -1;
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                const int32_t t = pc[c] >> 2, j = pc[c] & 3;
+                // position 0 is low in slots 0-2, position 1 in slot 3
+                const bool on = pc[c] >= 0 && j <= 1;
+#pragma unroll
+                for (int m = 0; m < 3; m++) {
+                    const bool use = on && (j == 0 || m == 0);
+                    const int64_t pos = 4 * (int64_t)t + (j == 0 ? m : 3);
+                    ff[c][m] = use ? ffirst[pos] : -1;
+                    id[c][m] = use && FILL ? tf[pos] : 0;
                 }
-                n++;
+                q[c] = on && FILL ? tv4[t] : make_int4(0, 0, 0, 0);
+            }
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                const int32_t t = pc[c] >> 2, j = pc[c] & 3;
+#pragma unroll
+                for (int m = 0; m < 3; m++) {
+                    const int sl = j == 0 ? m : 3;
+                    if (ff[c][m] != 4 * t + sl || ff[c][m] < 0) continue;
+                    if (FILL) {
+                        int lo, mid, hi;
+                        face_pos(sl, lo, mid, hi);
+                        row[3 * n] = corner(q[c], mid);
+                        row[3 * n + 1] = corner(q[c], hi);
+                        row[3 * n + 2] = id[c][m];
+                    }
+                    n++;
+                }
             }
         }
         if (!FILL) cnt_or_off[a] = n;
