@@ -253,49 +253,17 @@ __global__ __launch_bounds__(VE_THREADS) void k_ve_fill(const int32_t *__restric
     }
 }
 
-// lower entries (u < v) take the id stored in u's row.  Eight entries at a
-// time: their row bounds, the fixed-step searches (rows under 128 entries)
-// and the id loads are issued together, so a vertex costs about ten
-// dependent loads instead of ten per entry.
+// lower entries (u < v) take the id stored in u's row
 __global__ void k_edge_lower(const int32_t *__restrict__ vv_off, const int32_t *__restrict__ vv,
                              int64_t V, int32_t *__restrict__ vv_eid) {
-    constexpr int B = 8;
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
          v += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t vb = vv_off[v], ve = vv_off[v + 1];
-        for (int32_t i0 = vb; i0 < ve; i0 += B) {
-            int32_t u[B], ub[B], un[B], lo[B];
-            bool on[B];
-#pragma unroll
-            for (int c = 0; c < B; c++) {
-                u[c] = i0 + c < ve ? vv[i0 + c] : INT32_MAX;
-                on[c] = u[c] < v;
-            }
-#pragma unroll
-            for (int c = 0; c < B; c++) {
-                ub[c] = on[c] ? vv_off[u[c]] : 0;
-                un[c] = on[c] ? vv_off[u[c] + 1] - ub[c] : 0;
-                lo[c] = 0;
-            }
-#pragma unroll
-            for (int step = 64; step >= 1; step >>= 1)
-#pragma unroll
-                for (int c = 0; c < B; c++) {
-                    const int32_t probe = lo[c] + step - 1;
-                    if (on[c] && un[c] < 128 && probe < un[c] && vv[ub[c] + probe] < v)
-                        lo[c] += step;
-                }
-#pragma unroll
-            for (int c = 0; c < B; c++) {
-                if (!on[c]) continue;
-                const int32_t j = un[c] < 128
-                                      ? (lo[c] < un[c] && vv[ub[c] + lo[c]] == v ? lo[c] : -1)
-                                      : bsearch_i32(vv + ub[c], un[c], (int32_t)v);
-                lo[c] = j >= 0 ? vv_eid[ub[c] + j] : -1;
-            }
-#pragma unroll
-            for (int c = 0; c < B; c++)
-                if (on[c]) vv_eid[i0 + c] = lo[c];
+        for (int32_t i = vv_off[v]; i < vv_off[v + 1]; i++) {
+            const int32_t u = vv[i];
+            if (u > v) continue;
+            const int32_t ub = vv_off[u];
+            const int32_t j = bsearch_i32(vv + ub, vv_off[u + 1] - ub, (int32_t)v);
+            vv_eid[i] = j >= 0 ? vv_eid[ub + j] : -1;
         }
     }
 }
