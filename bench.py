@@ -379,6 +379,17 @@ def main() -> int:
     d.sync()
     ms = start.elapsed_time(stop) / args.steps
 
+    # K more steps with an event after each: the median step (the spread
+    # behind the mean above)
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    evs[0].record(stream)
+    for i in range(args.steps):
+        d.vv_critpoints()
+        evs[i + 1].record(stream)
+    torch.cuda.synchronize()
+    per_step = sorted(evs[i].elapsed_time(evs[i + 1]) for i in range(args.steps))
+    ms_median = per_step[len(per_step) // 2]
+
     # per-stage split of one more step (events between launches)
     d.profile(True)
     d.vv_critpoints()
@@ -451,6 +462,8 @@ def main() -> int:
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms, 4),
+        "ms_per_step_median": round(ms_median, 4),
+        "ms_per_step_min_max": [round(per_step[0], 4), round(per_step[-1], 4)],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
