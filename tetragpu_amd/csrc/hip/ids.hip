@@ -218,7 +218,7 @@ __global__ __launch_bounds__(VE_THREADS) void k_ve_fill(const int32_t *__restric
     __shared__ int64_t col[VE_CAP][VE_THREADS];  // (id << 32) | hi
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
          v += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t rb = vv_off[v], re = vv_off[v + 1];
+        const int32_t re = vv_off[v + 1];
         const int32_t n = off[v + 1] - off[v];
         int32_t *row = out + 2 * (int64_t)off[v];
         const int32_t ub = re - n;  // first upper entry
@@ -946,7 +946,6 @@ int ensure_edges(tg_mesh &m) {
     if ((rc = m.ev.ensure(I * 2 * (size_t)std::max<int64_t>(E, 1))) ||
         (rc = m.ve_vals.ensure(I * 2 * (size_t)std::max<int64_t>(E, 1))))
         return rc;
-    const int4 *tv4 = m.tv.as<const int4>();
     int32_t *voff = m.ve_off.as<int32_t>();
     if (V > 0) {
         hipLaunchKernelGGL(k_ve_count, dim3(grid(V)), dim3(256), 0, m.stream,
