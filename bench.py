@@ -430,11 +430,15 @@ def main() -> int:
         from oracle import binding as O
 
         threads = cpu_threads()
-        _, _, secs = O.vv_cp_omp(mesh.cells, V, mesh.scalars, threads)
+        ref_vv, ref_cls, secs = O.vv_cp_omp(mesh.cells, V, mesh.scalars, threads)
+        off, vals = d.relation("VV")
+        same = bool(np.array_equal(off, ref_vv.off) and np.array_equal(vals[:, 0], ref_vv.val[:, 0])
+                    and np.array_equal(d.classes(), ref_cls))
         cpu = {"value": round(T / secs, 1), "unit": "tets/s", "cores": threads,
                "kind": "port",
                "sample": f"full workload ({T} tets), one pass of the OpenMP oracle "
-                         f"(VV + critical points), {secs:.2f} s"}
+                         f"(VV + critical points), {secs:.2f} s",
+               "gpu_identical_to_oracle": same}
     d.close()
 
     # end to end from host buffers: upload TV + scalars, run, copy VV + classes back
