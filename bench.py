@@ -34,6 +34,23 @@ def bytes_alg(T: int, V: int, E: int) -> int:
     return 16 * T + 4 * V + 2 * (4 * (V + 1) + 8 * E) + 12 * V
 
 
+def load_valu(workload: str):
+    p = ROOT / "profiles" / f"valu_{workload}.json"
+    if not p.exists():
+        return None
+    return json.loads(p.read_text())
+
+
+def valu_busy(workload: str, dom_ms: float):
+    v = load_valu(workload)
+    if not v or dom_ms <= 0:
+        return None
+    k = next((x for n, x in v["kernels"].items() if n.startswith("k_bkt_vv_cp")), None)
+    if not k:
+        return None
+    return round(k["SQ_INSTS_VALU"] * 4 / (dom_ms * 1e-3 * v["clock_hz"] * v["simds"]), 3)
+
+
 def load_traffic(workload: str):
     p = ROOT / "profiles" / f"traffic_{workload}.json"
     if not p.exists():
@@ -420,6 +437,9 @@ def main() -> int:
             "traffic": traffic.get("dominant_kernel_bytes") if traffic else None,
             "traffic_gbs": (round(traffic["dominant_kernel_bytes"] / (dom_ms * 1e-3) / 1e9, 1)
                             if traffic and dom_ms > 0 else None),
+            # share of all SIMD cycles spent issuing VALU work (instruction
+            # count from rocprofv3, duration live): the kernel's compute bound
+            "valu_busy_frac": valu_busy(workload, dom_ms),
         },
         "stages_ms": {k: round(v, 4) for k, v in stages},
     }
