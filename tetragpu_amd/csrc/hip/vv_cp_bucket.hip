@@ -1285,12 +1285,16 @@ __global__ __launch_bounds__(HV_THREADS) void k_bkt_heavy(
 }
 
 // ---- VT rows from the routed corner records -------------------------------------
+// VT[v] lists the cells holding v in ascending cell order, as the reference's
+// elective_make_VT appends them (src/cpu_extraction.cpp:291-342); here as
+// corner codes 4t + j, so the row order is the cell order.
 // One workgroup per bucket.  The bucket's records (v, 4t + j) are counted per
-// vertex, placed per row (LDS, or the dead level-1 array `stage` for buckets
-// with more than VT_CAP records), and every entry then takes its place in
-// its ascending row by counting the smaller codes of that row (codes in a
-// row are distinct).  The bucket's rows fill the same range of the CSR that
-// its records fill in bucket order, so row offsets need no global scan.
+// vertex and placed per row (LDS, or the dead level-1 array `stage` for
+// buckets with more than VT_CAP records).  Rows of at most 32 entries are
+// sorted in registers, one thread per vertex; every entry of a longer row
+// takes its place by counting the smaller codes of that row (codes in a row
+// are distinct).  The bucket's rows fill the same range of the CSR that its
+// records fill in bucket order, so row offsets need no global scan.
 constexpr int VT_THREADS = 256, VT_CAP = 8192;
 
 template <int BV, typename P>
