@@ -243,6 +243,42 @@ def bench_relations(t, torch, mesh) -> dict:
     return out
 
 
+def bench_small(t, torch, steps: int = 200, check: bool = True) -> dict:
+    """Kuhn 24^3 (82,944 tets) with U[0,1) scalars and seeded permutations:
+    the CPU-runnable parity case.  It fits in L2/MALL, so it is latency-bound;
+    reported as tets/s per step (mean over `steps` back-to-back steps)."""
+    import numpy as np
+
+    m = t.kuhn(24, seed=1234, scalar="uniform")
+    stream = torch.cuda.Stream()
+    out = {"workload": "kuhn24_uniform", "tets": m.n_cells, "vertices": m.n_points}
+    with t.Mesh(m.cells, m.n_points, m.scalars, stream=stream.cuda_stream) as d:
+        for _ in range(5):
+            d.vv_critpoints()
+        d.sync()
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(steps):
+            d.vv_critpoints()
+        b.record(stream)
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / steps
+        cls = d.classes()
+        off, vals = d.relation("VV")
+    out.update({"ms_per_step": round(ms, 4), "tets_per_s": round(m.n_cells / (ms * 1e-3), 1)})
+    if check:
+        from oracle import binding as O
+
+        VV, ref, _ = O.vv_cp_omp(m.cells, m.n_points, m.scalars, cpu_threads())
+        out["identical_to_oracle"] = bool(np.array_equal(VV.off, off)
+                                          and np.array_equal(VV.val[:, 0], vals[:, 0])
+                                          and np.array_equal(ref, cls))
+    log(f"small: {out}")
+    return out
+
+
 def bench_irregular(t, torch, steps: int, seed: int = 5, check: bool = True) -> dict:
     """Kuhn 96^3 with 15 % of the tets stellar-split 1->4 at their centroid,
     plus 32 hub vertices each raised by 64 repeated splits around them, so
@@ -473,6 +509,7 @@ def main() -> int:
     partitioned = None if args.no_partitions else bench_partitioned(t, mesh, whole_cls)
 
     relations = None if args.no_relations else bench_relations(t, torch, mesh)
+    small = bench_small(t, torch, check=not args.no_cpu)
     irregular = None if args.no_irregular else bench_irregular(t, torch, max(args.steps // 2, 1),
                                                                 check=not args.no_cpu)
     batch = bench_batch(t, torch, args.batch_meshes, 3) if args.batch_meshes > 0 else None
@@ -501,6 +538,7 @@ def main() -> int:
         "end_to_end_ms": round(e2e_ms, 2),
         "end_to_end_note": "host cells/scalars in, VV CSR + classes out, incl. allocation (first use)",
         "relations": relations,
+        "small": small,
         "irregular": irregular,
         "partitioned": partitioned,
         "batched": batch,
