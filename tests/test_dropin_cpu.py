@@ -350,3 +350,12 @@ def test_cpu_partitioned_critpoints_equal_oracle(probe, tmp_path, case):
     _, cells, V, s, parts, pc = case
     got = probe_part(probe, tmp_path, cells, V, s, parts, "cpu", pc)
     assert np.array_equal(got, O.critpoints_partitioned(cells, V, s, parts, pc))
+
+
+def test_partitioned_classes_rejects_bad_ids_before_any_device_call():
+    # argument checks of tetra_partitioned_classes run on the host first
+    with pytest.raises(ValueError, match="negative partition id at point 2"):
+        t.partitioned_classes(np.array([[0, 1, 2, 3]], np.int32), 4, np.zeros(4),
+                              np.array([0, 0, -1, 0]))
+    with pytest.raises(ValueError, match="one entry per point"):
+        t.partitioned_classes(np.array([[0, 1, 2, 3]], np.int32), 4, np.zeros(3), np.zeros(4))
