@@ -316,14 +316,18 @@ __global__ __launch_bounds__(ET_THREADS) void k_et(const int32_t *__restrict__ e
             for (int32_t k = 0; k < nb; k++) buf[na + k][threadIdx.x] = vtc[j + k] >> 2;
             i = 0, ie = na, j = na, je = na + nb;
             int32_t x = i < ie ? buf[i][threadIdx.x] : INF, y = j < je ? buf[j][threadIdx.x] : INF;
-            while (x != INF && y != INF) {
+            const int32_t last = max(je - 1, 0);
+            while (x != INF && y != INF) {  // branch-free advance (two reads per step)
                 if (x == y) {
                     row[n] = x;
                     n++;
                 }
                 const int32_t m = min(x, y);
-                if (x == m) x = ++i < ie ? buf[i][threadIdx.x] : INF;
-                if (y == m) y = ++j < je ? buf[j][threadIdx.x] : INF;
+                i += x == m;
+                j += y == m;
+                const int32_t xa = buf[min(i, last)][threadIdx.x], yb = buf[min(j, last)][threadIdx.x];
+                x = i < ie ? xa : INF;
+                y = j < je ? yb : INF;
             }
         } else {
             int32_t x = i < ie ? vtc[i] >> 2 : INF, y = j < je ? vtc[j] >> 2 : INF;
@@ -599,29 +603,48 @@ __global__ __launch_bounds__(ET_THREADS) void k_ef(const int64_t *__restrict__ e
         const bool lds = 2 * (b1 - b0) <= EF_CAP;
         int32_t *grow = tmp + 2 * b0;
         int64_t m = 0;
-        for (int64_t i = b0; i < b1; i++) {
-            const int32_t t = et[i];
-            int k = 0;
-            for (; k < 6; k++)
-                if (te[6 * (int64_t)t + k] == (int32_t)e) break;
-            // corner pair of edge slot k; faces holding both corners
-            const int pi[6] = {0, 0, 0, 1, 1, 2}, pj[6] = {1, 2, 3, 2, 3, 3};
-            for (int s = 0; s < 4; s++) {
-                int lo, mid, hi;
-                face_pos(s, lo, mid, hi);
-                const bool has_i = lo == pi[k] || mid == pi[k] || hi == pi[k];
-                const bool has_j = lo == pj[k] || mid == pj[k] || hi == pj[k];
-                if (!(has_i && has_j)) continue;
-                const int32_t x = tf[4 * (int64_t)t + s];
-                if (lds) {  // insert into the ascending column unless present
-                    int64_t p = m;
-                    while (p > 0 && col[p - 1][threadIdx.x] > x) p--;
-                    if (p > 0 && col[p - 1][threadIdx.x] == x) continue;
-                    for (int64_t q = m; q > p; q--) col[q][threadIdx.x] = col[q - 1][threadIdx.x];
-                    col[p][threadIdx.x] = x;
-                    m++;
-                } else {
-                    insert_sorted(grow, m, x);
+        // four cells at a time: their TE and TF rows are loaded together
+        // (unconditional loads from a clamped cell index)
+        for (int64_t i0 = b0; i0 < b1; i0 += 4) {
+            int32_t tc[4];
+            int2 r01[4], r23[4], r45[4];
+            int4 fr[4];
+#pragma unroll
+            for (int c = 0; c < 4; c++) tc[c] = et[min(i0 + c, b1 - 1)];
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const int2 *tr = reinterpret_cast<const int2 *>(te + 6 * (int64_t)tc[c]);
+                r01[c] = tr[0];
+                r23[c] = tr[1];
+                r45[c] = tr[2];
+                fr[c] = reinterpret_cast<const int4 *>(tf)[tc[c]];
+            }
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                if (i0 + c >= b1) break;
+                const int32_t ee = (int32_t)e;
+                const int k = r01[c].x == ee ? 0 : r01[c].y == ee ? 1 : r23[c].x == ee ? 2
+                            : r23[c].y == ee ? 3 : r45[c].x == ee ? 4 : r45[c].y == ee ? 5 : 6;
+                if (k == 6) continue;  // TE and ET disagree: no face of this cell
+                // corner pair of edge slot k; faces holding both corners
+                const int pi[6] = {0, 0, 0, 1, 1, 2}, pj[6] = {1, 2, 3, 2, 3, 3};
+                for (int s = 0; s < 4; s++) {
+                    int lo, mid, hi;
+                    face_pos(s, lo, mid, hi);
+                    const bool has_i = lo == pi[k] || mid == pi[k] || hi == pi[k];
+                    const bool has_j = lo == pj[k] || mid == pj[k] || hi == pj[k];
+                    if (!(has_i && has_j)) continue;
+                    const int32_t x = s == 0 ? fr[c].x : s == 1 ? fr[c].y : s == 2 ? fr[c].z : fr[c].w;
+                    if (lds) {  // insert into the ascending column unless present
+                        int64_t p = m;
+                        while (p > 0 && col[p - 1][threadIdx.x] > x) p--;
+                        if (p > 0 && col[p - 1][threadIdx.x] == x) continue;
+                        for (int64_t q = m; q > p; q--) col[q][threadIdx.x] = col[q - 1][threadIdx.x];
+                        col[p][threadIdx.x] = x;
+                        m++;
+                    } else {
+                        insert_sorted(grow, m, x);
+                    }
                 }
             }
         }
@@ -697,13 +720,24 @@ __global__ __launch_bounds__(ROW_THREADS) void k_tt(const int4 *__restrict__ tv4
             int32_t h1 = a1 < z1 ? buf[a1][threadIdx.x] : INF;
             int32_t h2 = a2 < z2 ? buf[a2][threadIdx.x] : INF;
             int32_t h3 = a3 < z3 ? buf[a3][threadIdx.x] : INF;
+            // branch-free advance: every head re-reads its (clamped) position,
+            // so the four LDS reads of an iteration issue together
+            const int32_t last = max(z3 - 1, 0);
             while (true) {
                 const int32_t mn = min(min(h0, h1), min(h2, h3));
                 if (mn == INF) break;
-                if (h0 == mn) h0 = ++a0 < z0 ? buf[a0][threadIdx.x] : INF;
-                if (h1 == mn) h1 = ++a1 < z1 ? buf[a1][threadIdx.x] : INF;
-                if (h2 == mn) h2 = ++a2 < z2 ? buf[a2][threadIdx.x] : INF;
-                if (h3 == mn) h3 = ++a3 < z3 ? buf[a3][threadIdx.x] : INF;
+                a0 += h0 == mn;
+                a1 += h1 == mn;
+                a2 += h2 == mn;
+                a3 += h3 == mn;
+                const int32_t y0 = buf[min(a0, last)][threadIdx.x];
+                const int32_t y1 = buf[min(a1, last)][threadIdx.x];
+                const int32_t y2 = buf[min(a2, last)][threadIdx.x];
+                const int32_t y3 = buf[min(a3, last)][threadIdx.x];
+                h0 = a0 < z0 ? y0 : INF;
+                h1 = a1 < z1 ? y1 : INF;
+                h2 = a2 < z2 ? y2 : INF;
+                h3 = a3 < z3 ? y3 : INF;
                 if (mn == (int32_t)t) continue;
                 row[n] = mn;
                 n++;
@@ -811,10 +845,14 @@ __global__ __launch_bounds__(ROW_THREADS) void k_ee(const int32_t *__restrict__ 
             stage_row(buf, na, eids + j, nb, 0);
             i = 0, ie = na, j = na, je = na + nb;
             int32_t x = i < ie ? buf[i][threadIdx.x] : INF, y = j < je ? buf[j][threadIdx.x] : INF;
-            while (x != INF || y != INF) {
+            const int32_t last = max(je - 1, 0);
+            while (x != INF || y != INF) {  // branch-free advance (two reads per step)
                 const int32_t mn = min(x, y);
-                if (x == mn) x = ++i < ie ? buf[i][threadIdx.x] : INF;
-                if (y == mn) y = ++j < je ? buf[j][threadIdx.x] : INF;
+                i += x == mn;
+                j += y == mn;
+                const int32_t xa = buf[min(i, last)][threadIdx.x], yb = buf[min(j, last)][threadIdx.x];
+                x = i < ie ? xa : INF;
+                y = j < je ? yb : INF;
                 if (mn != (int32_t)e) row[m++] = mn;
             }
         } else {
