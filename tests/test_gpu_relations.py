@@ -177,3 +177,12 @@ def test_vt_bucketed_routes(gpu, case, route, bv):
         d.build(gpu.flags("VT"))
         same_csr(d.relation("VT"), O.vt(cells, V), "VT")
         assert d.approx_max_vv() == O.approx_max_vv(cells, V)
+
+
+def test_duplicated_cell_faces_and_ff(gpu):
+    # two cells on the same four vertices share all four faces: FT holds both,
+    # and FF(f) is the other three faces once each
+    cells = np.array([[0, 1, 2, 3], [0, 1, 2, 3], [4, 1, 2, 5]], np.int32)
+    ref = check_all(gpu, cells, 6)
+    assert ref["FT"] is not None
+    assert [len(r) for r in np.split(ref["FF"].val[:, 0], ref["FF"].off[1:-1])][:4] == [3, 3, 3, 3]

@@ -795,32 +795,43 @@ __global__ void k_ff(const int32_t *__restrict__ ft, int64_t F, const int32_t *_
                      int64_t *__restrict__ cnt_or_off, int32_t *__restrict__ out) {
     for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < F;
          f += (int64_t)gridDim.x * blockDim.x) {
-        int32_t c[8];
-        int n = 0;
-        for (int k = 0; k < 2; k++) {
-            const int32_t t = ft[2 * f + k];
-            if (t < 0) continue;
-            for (int s = 0; s < 4; s++) {
-                const int32_t g = tf[4 * (int64_t)t + s];
-                if (g == (int32_t)f) continue;
-                int i = n - 1;
-                bool dup = false;
-                for (int z = 0; z < n; z++) dup |= c[z] == g;
-                if (dup) continue;
-                while (i >= 0 && c[i] > g) {
-                    c[i + 1] = c[i];
-                    i--;
+        // the faces of f's (at most two) cells other than f: both TF rows are
+        // loaded at once, the six candidates sorted by a network, then the
+        // distinct ones counted or written in order
+        const int2 cells = reinterpret_cast<const int2 *>(ft)[f];
+        const int4 r0 = reinterpret_cast<const int4 *>(tf)[cells.x >= 0 ? cells.x : 0];
+        const int4 r1 = reinterpret_cast<const int4 *>(tf)[cells.y >= 0 ? cells.y : 0];
+        const int32_t me = (int32_t)f;
+        int32_t c[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const bool on = i < 4 ? cells.x >= 0 : cells.y >= 0;
+            if (!on || c[i] == me) c[i] = INT32_MAX;
+        }
+        // bitonic network over eight registers (ascending)
+#pragma unroll
+        for (int k = 2; k <= 8; k <<= 1)
+#pragma unroll
+            for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    const int l = i ^ j;
+                    if (l > i) {
+                        const int32_t a = c[i], b = c[l];
+                        const bool up = (i & k) == 0;
+                        c[i] = up ? min(a, b) : max(a, b);
+                        c[l] = up ? max(a, b) : min(a, b);
+                    }
                 }
-                c[i + 1] = g;
-                n++;
-            }
+        int64_t o = FILL ? cnt_or_off[f] : 0;
+        int n = 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const bool keep = c[i] != INT32_MAX && (i == 0 || c[i] != c[i - 1]);
+            if (FILL && keep) out[o + n] = c[i];
+            n += keep;
         }
-        if (FILL) {
-            const int64_t o = cnt_or_off[f];
-            for (int i = 0; i < n; i++) out[o + i] = c[i];
-        } else {
-            cnt_or_off[f] = n;
-        }
+        if (!FILL) cnt_or_off[f] = n;
     }
 }
 
