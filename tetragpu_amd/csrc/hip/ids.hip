@@ -176,13 +176,33 @@ __global__ void k_edge_ids(const int32_t *__restrict__ vv_off, const int32_t *__
                            int32_t *__restrict__ vv_eid, int32_t *__restrict__ ev) {
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
          v += (int64_t)gridDim.x * blockDim.x) {
-        for (int32_t i = vv_off[v]; i < vv_off[v + 1]; i++) {
-            const int32_t u = vv[i];
-            if (u < v || efirst[i] == INF) continue;
-            const int32_t id = rank_of(bits, wpre, efirst[i]);
-            vv_eid[i] = id;
-            ev[2 * (int64_t)id] = (int32_t)v;
-            ev[2 * (int64_t)id + 1] = u;
+        const int32_t rb = vv_off[v], re = vv_off[v + 1];
+        // eight entries at a time: unconditional loads from clamped indices
+        // (the rank words of an unused lane are those of the row's first
+        // entry), then the writes of the upper entries
+        for (int32_t i0 = rb; i0 < re; i0 += 8) {
+            int32_t u[8], p[8], id[8];
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                const int32_t i = min(i0 + c, re - 1);
+                u[c] = vv[i];
+                p[c] = efirst[i];
+            }
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                const int on = (int)(i0 + c < re) & (int)(u[c] > v) & (int)(p[c] != INF);
+                const int32_t q = on ? p[c] : 0;
+                const uint32_t w = bits[q >> 5];
+                const int32_t pre = wpre[q >> 5];
+                id[c] = on ? pre + __popc(w & ((1u << (q & 31)) - 1u)) : -1;
+            }
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                if (id[c] < 0) continue;
+                vv_eid[i0 + c] = id[c];
+                ev[2 * (int64_t)id[c]] = (int32_t)v;
+                ev[2 * (int64_t)id[c] + 1] = u[c];
+            }
         }
     }
 }
