@@ -383,6 +383,9 @@ def main() -> int:
                     help="skip the irregular-degree mesh (Kuhn 96^3, 15 %% split)")
     ap.add_argument("--no-partitions", action="store_true",
                     help="skip the partition loop (8 z slabs of the benchmark mesh)")
+    ap.add_argument("--profile-relations", action="store_true",
+                    help="only build every relation flag + critical points on two fresh meshes "
+                         "of the workload (for a kernel trace) and exit")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (reference kernel for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -407,6 +410,16 @@ def main() -> int:
     workload = f"kuhn{n}_gaussians"
     t0 = time.time()
     mesh = t.kuhn(n, seed=args.seed, scalar="gaussians")
+    if args.profile_relations:
+        for _ in range(2):
+            with t.Mesh(mesh.cells, mesh.n_points, mesh.scalars) as d:
+                d.sync()
+                t1 = time.perf_counter()
+                d.build(t.ALL_FLAGS)
+                d.vv_critpoints()
+                d.sync()
+                log(f"all flags + critical points {(time.perf_counter() - t1) * 1e3:.2f} ms")
+        return 0
     gen_s = time.time() - t0
     T, V = mesh.n_cells, mesh.n_points
     # a dedicated default-priority stream: the library launches on it and the
