@@ -700,16 +700,66 @@ __device__ __forceinline__ void stage_row(int32_t (*buf)[ROW_THREADS], int at,
     for (int32_t k = 0; k < n; k++) buf[at + k][threadIdx.x] = src[k] >> shift;
 }
 
+// Unaligned 16- and 8-byte loads: dword alignment is enough for
+// global_load_dwordx4/x2 on gfx950.
+typedef int32_t i32x4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef int32_t i32x2u __attribute__((ext_vector_type(2), aligned(4)));
+
+// stage_row by 16-byte loads.  Whole quads are read, up to three words past
+// the row (the source array carries that slack), and all four words go to
+// the column: the words past the row land where the next row is staged, or
+// in the column's three spare slots after the last row.
+__device__ __forceinline__ void stage_quads(int32_t (*buf)[ROW_THREADS], int at,
+                                            const int32_t *__restrict__ src, int32_t n,
+                                            int shift) {
+#pragma unroll 4
+    for (int32_t k = 0; k < n; k += 4) {
+        const i32x4u v = *reinterpret_cast<const i32x4u *>(src + k);
+        buf[at + k][threadIdx.x] = v.x >> shift;
+        buf[at + k + 1][threadIdx.x] = v.y >> shift;
+        buf[at + k + 2][threadIdx.x] = v.z >> shift;
+        buf[at + k + 3][threadIdx.x] = v.w >> shift;
+    }
+}
+
+// A thread's output row written in 16-byte stores: words gather in a
+// register quad, stored when full; the last, partial quad is stored whole,
+// so the row's slot must start 16-byte aligned and span a multiple of four
+// words.  One store per four words instead of one per word: scattered
+// per-thread rows cost the vector memory path per store instruction.
+struct QuadOut {
+    int32_t *row;
+    int64_t n = 0;
+    int4 q = make_int4(0, 0, 0, 0);
+    __device__ __forceinline__ void put(int32_t v) {
+        const int k = (int)(n & 3);
+        q.x = k == 0 ? v : q.x;
+        q.y = k == 1 ? v : q.y;
+        q.z = k == 2 ? v : q.z;
+        q.w = k == 3 ? v : q.w;
+        if (k == 3) *reinterpret_cast<int4 *>(row + n - 3) = q;
+        n++;
+    }
+    __device__ __forceinline__ void flush() {
+        if (n & 3) *reinterpret_cast<int4 *>(row + (n & ~(int64_t)3)) = q;
+    }
+};
+
 // TT(t): union of VT rows of t's corners, minus t (4-way merge), written in
-// one pass to a padded slot of |VT rows| - 4 words at pad[t], with its
-// length in cnt[t]; k_pack_rows packs the rows after the scan of cnt.
+// one pass to a padded slot at pad[t] of |VT rows| - 4 words rounded up to a
+// multiple of four, with its length in cnt[t]; the rows are packed after
+// the scan of cnt.
 __global__ void k_tt_bound(const int4 *__restrict__ tv4, int64_t T,
                            const int32_t *__restrict__ vt_off, int64_t *__restrict__ ub) {
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < T;
          t += (int64_t)gridDim.x * blockDim.x) {
         const int4 q = tv4[t];
-        ub[t] = (int64_t)(vt_off[q.x + 1] - vt_off[q.x]) + (vt_off[q.y + 1] - vt_off[q.y]) +
-                (vt_off[q.z + 1] - vt_off[q.z]) + (vt_off[q.w + 1] - vt_off[q.w]) - 4;
+        const i32x2u r0 = *reinterpret_cast<const i32x2u *>(vt_off + q.x);
+        const i32x2u r1 = *reinterpret_cast<const i32x2u *>(vt_off + q.y);
+        const i32x2u r2 = *reinterpret_cast<const i32x2u *>(vt_off + q.z);
+        const i32x2u r3 = *reinterpret_cast<const i32x2u *>(vt_off + q.w);
+        const int64_t n = (int64_t)(r0.y - r0.x) + (r1.y - r1.x) + (r2.y - r2.x) + (r3.y - r3.x) - 4;
+        ub[t] = (n + 3) & ~(int64_t)3;
     }
 }
 
@@ -719,23 +769,27 @@ __global__ __launch_bounds__(ROW_THREADS) void k_tt(const int4 *__restrict__ tv4
                                                     const int64_t *__restrict__ pad,
                                                     int64_t *__restrict__ cnt,
                                                     int32_t *__restrict__ out) {
-    __shared__ int32_t buf[ROW_CAP][ROW_THREADS];
+    __shared__ int32_t buf[ROW_CAP + 3][ROW_THREADS];  // three spare slots for stage_quads
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < T;
          t += (int64_t)gridDim.x * blockDim.x) {
         const int4 q = tv4[t];
-        int32_t p0 = vt_off[q.x], e0 = vt_off[q.x + 1], p1 = vt_off[q.y], e1 = vt_off[q.y + 1];
-        int32_t p2 = vt_off[q.z], e2 = vt_off[q.z + 1], p3 = vt_off[q.w], e3 = vt_off[q.w + 1];
-        int64_t n = 0;
-        int32_t *row = out + pad[t];
+        const i32x2u r0 = *reinterpret_cast<const i32x2u *>(vt_off + q.x);
+        const i32x2u r1 = *reinterpret_cast<const i32x2u *>(vt_off + q.y);
+        const i32x2u r2 = *reinterpret_cast<const i32x2u *>(vt_off + q.z);
+        const i32x2u r3 = *reinterpret_cast<const i32x2u *>(vt_off + q.w);
+        int32_t p0 = r0.x, e0 = r0.y, p1 = r1.x, e1 = r1.y;
+        int32_t p2 = r2.x, e2 = r2.y, p3 = r3.x, e3 = r3.y;
+        QuadOut o;
+        o.row = out + pad[t];
         const int32_t n0 = e0 - p0, n1 = e1 - p1, n2 = e2 - p2, n3 = e3 - p3;
         if (n0 + n1 + n2 + n3 <= ROW_CAP) {
-            stage_row(buf, 0, vtc + p0, n0, 2);
-            stage_row(buf, n0, vtc + p1, n1, 2);
-            stage_row(buf, n0 + n1, vtc + p2, n2, 2);
-            stage_row(buf, n0 + n1 + n2, vtc + p3, n3, 2);
             // heads as LDS positions
             int32_t a0 = 0, a1 = n0, a2 = n0 + n1, a3 = n0 + n1 + n2;
             const int32_t z0 = n0, z1 = n0 + n1, z2 = n0 + n1 + n2, z3 = n0 + n1 + n2 + n3;
+            stage_quads(buf, 0, vtc + p0, n0, 2);
+            stage_quads(buf, z0, vtc + p1, n1, 2);
+            stage_quads(buf, z1, vtc + p2, n2, 2);
+            stage_quads(buf, z2, vtc + p3, n3, 2);
             int32_t h0 = a0 < z0 ? buf[a0][threadIdx.x] : INF;
             int32_t h1 = a1 < z1 ? buf[a1][threadIdx.x] : INF;
             int32_t h2 = a2 < z2 ? buf[a2][threadIdx.x] : INF;
@@ -759,8 +813,7 @@ __global__ __launch_bounds__(ROW_THREADS) void k_tt(const int4 *__restrict__ tv4
                 h2 = a2 < z2 ? y2 : INF;
                 h3 = a3 < z3 ? y3 : INF;
                 if (mn == (int32_t)t) continue;
-                row[n] = mn;
-                n++;
+                o.put(mn);
             }
         } else {
             int32_t h0 = p0 < e0 ? vtc[p0] >> 2 : INF, h1 = p1 < e1 ? vtc[p1] >> 2 : INF;
@@ -773,11 +826,11 @@ __global__ __launch_bounds__(ROW_THREADS) void k_tt(const int4 *__restrict__ tv4
                 if (h2 == mn) h2 = ++p2 < e2 ? vtc[p2] >> 2 : INF;
                 if (h3 == mn) h3 = ++p3 < e3 ? vtc[p3] >> 2 : INF;
                 if (mn == (int32_t)t) continue;
-                row[n] = mn;
-                n++;
+                o.put(mn);
             }
         }
-        cnt[t] = n;
+        o.flush();
+        cnt[t] = o.n;
     }
 }
 
@@ -806,6 +859,48 @@ __global__ __launch_bounds__(256) void k_pack_rows(const int64_t *__restrict__ p
             else hi = mid - 1;
         }
         dst[o] = src[s_pad[lo] + (o - s_off[lo])];
+    }
+}
+
+// k_pack_rows for long rows (tens of words on average): one wave per 64
+// consecutive rows, the lanes copying a row 64 words at a time; the first
+// 64 words of four rows are loaded together.  Row bounds are read once per
+// wave, one row per lane, and broadcast with readlane.
+__device__ __forceinline__ int64_t readlane64(int64_t v, int lane) {
+    const int32_t lo = __builtin_amdgcn_readlane((int32_t)(uint32_t)v, lane);
+    const int32_t hi = __builtin_amdgcn_readlane((int32_t)(v >> 32), lane);
+    return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+__global__ __launch_bounds__(256) void k_pack_long_rows(const int64_t *__restrict__ pad,
+                                                        const int64_t *__restrict__ off,
+                                                        int64_t rows,
+                                                        const int32_t *__restrict__ src,
+                                                        int32_t *__restrict__ dst) {
+    const int lane = threadIdx.x & 63;
+    const int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
+    if (r0 >= rows) return;  // the whole wave
+    const int nr = (int)min<int64_t>(64, rows - r0);
+    const int64_t mp = lane < nr ? pad[r0 + lane] : 0;
+    const int64_t mo = lane < nr ? off[r0 + lane] : 0;
+    const int32_t ml = lane < nr ? (int32_t)(off[r0 + lane + 1] - mo) : 0;
+    for (int i = 0; i < nr; i += 4) {
+        int64_t p[4], o[4];
+        int32_t n[4], v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int r = min(i + u, nr - 1);
+            p[u] = readlane64(mp, r);
+            o[u] = readlane64(mo, r);
+            n[u] = i + u < nr ? __builtin_amdgcn_readlane(ml, r) : 0;
+            // clamped to the row (or to word 0 for an empty row): every load is in bounds
+            v[u] = src[n[u] > 0 ? p[u] + min(lane, n[u] - 1) : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (lane < n[u]) dst[o[u] + lane] = v[u];
+            for (int32_t k = 64 + lane; k < n[u]; k += 64) dst[o[u] + k] = src[p[u] + k];
+        }
     }
 }
 
@@ -1224,10 +1319,10 @@ int ensure_tt(tg_mesh &m) {
     if ((rc = scan_counts64(m, off, T, nnz))) return rc;
     if ((rc = m.tt_vals.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)))) return rc;
     if (T > 0) {
-        hipLaunchKernelGGL(k_pack_rows, dim3((unsigned)((T + 255) / 256)), dim3(256), 0,
+        hipLaunchKernelGGL(k_pack_long_rows, dim3((unsigned)((T + 255) / 256)), dim3(256), 0,
                            m.stream, pad, off, T, m.tmp_c.as<int32_t>(),
                            m.tt_vals.as<int32_t>());
-        TG_LAUNCH_CHECK("k_pack_rows");
+        TG_LAUNCH_CHECK("k_pack_long_rows");
     }
     m.tt_nnz = nnz;
     m.tt_built = true;

@@ -167,8 +167,9 @@ int prepare_vt(tg_mesh &m) {
     int rc = prepare_hot_path(m);
     if (rc) return rc;
     const size_t V = (size_t)m.n_points, T = (size_t)m.n_cells, I = sizeof(int32_t);
+    // vtc carries four words of slack: row staging reads whole 16-byte quads
     if ((rc = m.vt_cnt.ensure(I * V)) || (rc = m.vt_off.ensure(I * (V + 1))) ||
-        (rc = m.vt_cur.ensure(I * V)) || (rc = m.vtc.ensure(I * 4 * T)))
+        (rc = m.vt_cur.ensure(I * V)) || (rc = m.vtc.ensure(I * (4 * T + 4))))
         return rc;
     m.vt_ready = true;
     return OK;
