@@ -67,6 +67,82 @@ __device__ __forceinline__ int32_t bsearch_i32(const int32_t *a, int32_t n, int3
     return (lo < n && a[lo] == x) ? lo : -1;
 }
 
+// Unaligned 16- and 8-byte loads: dword alignment is enough for
+// global_load_dwordx4/x2 on gfx950.
+typedef int32_t i32x4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef int32_t i32x2u __attribute__((ext_vector_type(2), aligned(4)));
+
+// Merges of a few short sorted rows (ET, TT, EE) stage the rows in LDS
+// first: one column per thread (consecutive threads in consecutive banks),
+// filled by independent 16-byte loads, so a row costs one memory latency
+// instead of one per element; the merge then reads LDS.  Whole quads are
+// read, up to three words past the row (the source array carries that
+// slack), and all four words go to the column: the words past the row land
+// where the next row is staged, or in the column's three spare slots after
+// the last row.  A thread whose rows exceed the column merges from global
+// memory.
+template <int W>
+__device__ __forceinline__ void stage_quads(int32_t (*buf)[W], int at,
+                                            const int32_t *__restrict__ src, int32_t n,
+                                            int shift) {
+#pragma unroll 4
+    for (int32_t k = 0; k < n; k += 4) {
+        const i32x4u v = *reinterpret_cast<const i32x4u *>(src + k);
+        buf[at + k][threadIdx.x] = v.x >> shift;
+        buf[at + k + 1][threadIdx.x] = v.y >> shift;
+        buf[at + k + 2][threadIdx.x] = v.z >> shift;
+        buf[at + k + 3][threadIdx.x] = v.w >> shift;
+    }
+}
+
+// A thread's output row written in 16-byte stores: words gather in a
+// register quad, stored when full; the last, partial quad is stored whole,
+// so the row's slot must start 16-byte aligned and span a multiple of four
+// words.  One store per four words instead of one per word: scattered
+// per-thread rows cost the vector memory path per store instruction.
+struct QuadOut {
+    int32_t *row;
+    int64_t n = 0;
+    int4 q = make_int4(0, 0, 0, 0);
+    __device__ __forceinline__ void put(int32_t v) {
+        const int k = (int)(n & 3);
+        q.x = k == 0 ? v : q.x;
+        q.y = k == 1 ? v : q.y;
+        q.z = k == 2 ? v : q.z;
+        q.w = k == 3 ? v : q.w;
+        if (k == 3) *reinterpret_cast<int4 *>(row + n - 3) = q;
+        n++;
+    }
+    __device__ __forceinline__ void flush() {
+        if (n & 3) *reinterpret_cast<int4 *>(row + (n & ~(int64_t)3)) = q;
+    }
+};
+
+// QuadOut for a row at its final, exact position: full quads in unaligned
+// 16-byte stores, the last one to three words one by one (the words after
+// the row belong to the next row).
+struct QuadOutExact {
+    int32_t *row;
+    int64_t n = 0;
+    int4 q = make_int4(0, 0, 0, 0);
+    __device__ __forceinline__ void put(int32_t v) {
+        const int k = (int)(n & 3);
+        q.x = k == 0 ? v : q.x;
+        q.y = k == 1 ? v : q.y;
+        q.z = k == 2 ? v : q.z;
+        q.w = k == 3 ? v : q.w;
+        if (k == 3) *reinterpret_cast<i32x4u *>(row + n - 3) = i32x4u{q.x, q.y, q.z, q.w};
+        n++;
+    }
+    __device__ __forceinline__ void flush() {
+        const int k = (int)(n & 3);
+        int32_t *tail = row + (n & ~(int64_t)3);
+        if (k > 0) tail[0] = q.x;
+        if (k > 1) tail[1] = q.y;
+        if (k > 2) tail[2] = q.z;
+    }
+};
+
 // ---- edges --------------------------------------------------------------------
 // Lane per vertex v: first position of every upper edge (v, u>v), flag it.
 // Every pair of v's cells whose low vertex is v also records the VV index of
@@ -298,10 +374,11 @@ __global__ void k_te(int32_t *__restrict__ te, int64_t n, const int32_t *__restr
 }
 
 // ---- ET = VT(lo) n VT(hi) -------------------------------------------------------
-// Both rows are staged in LDS when they fit (see stage_row), then merged in
-// one pass into a padded slot of min(|VT(lo)|, |VT(hi)|) words at pad[e],
-// with the row length in cnt[e]; k_pack_rows packs the rows afterwards.
-constexpr int ET_THREADS = 64, ET_CAP = 128;
+// Both rows are staged in LDS when they fit (stage_quads), then merged in
+// one pass into a padded slot at pad[e] of min(|VT(lo)|, |VT(hi)|) words
+// rounded up to a multiple of four (QuadOut), with the row length in
+// cnt[e]; k_pack_rows packs the rows afterwards.
+constexpr int ET_THREADS = 64, ET_CAP = 125;  // with stage_quads' three spare slots: 32 KB
 
 __global__ void k_et_bound(const int32_t *__restrict__ ev, int64_t E, int64_t V,
                            const int32_t *__restrict__ vt_off, int64_t *__restrict__ ub) {
@@ -309,7 +386,8 @@ __global__ void k_et_bound(const int32_t *__restrict__ ev, int64_t E, int64_t V,
          e += (int64_t)gridDim.x * blockDim.x) {
         const int32_t a = ev[2 * e], b = ev[2 * e + 1];
         const bool ok = a >= 0 && a < V && b >= 0 && b < V;  // always, for a built EV
-        ub[e] = ok ? min(vt_off[a + 1] - vt_off[a], vt_off[b + 1] - vt_off[b]) : 0;
+        const int64_t n = ok ? min(vt_off[a + 1] - vt_off[a], vt_off[b + 1] - vt_off[b]) : 0;
+        ub[e] = (n + 3) & ~(int64_t)3;
     }
 }
 
@@ -319,29 +397,24 @@ __global__ __launch_bounds__(ET_THREADS) void k_et(const int32_t *__restrict__ e
                                                    const int64_t *__restrict__ pad,
                                                    int64_t *__restrict__ cnt,
                                                    int32_t *__restrict__ out) {
-    __shared__ int32_t buf[ET_CAP][ET_THREADS];
+    __shared__ int32_t buf[ET_CAP + 3][ET_THREADS];
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
          e += (int64_t)gridDim.x * blockDim.x) {
         const int32_t a = ev[2 * e], b = ev[2 * e + 1];
         const bool ok = a >= 0 && a < V && b >= 0 && b < V;  // always, for a built EV
         int32_t i = ok ? vt_off[a] : 0, ie = ok ? vt_off[a + 1] : 0;
         int32_t j = ok ? vt_off[b] : 0, je = ok ? vt_off[b + 1] : 0;
-        int64_t n = 0;
-        int32_t *row = out + pad[e];
+        QuadOut o;
+        o.row = out + pad[e];
         const int32_t na = ie - i, nb = je - j;
         if (na + nb <= ET_CAP) {
-#pragma unroll 4
-            for (int32_t k = 0; k < na; k++) buf[k][threadIdx.x] = vtc[i + k] >> 2;
-#pragma unroll 4
-            for (int32_t k = 0; k < nb; k++) buf[na + k][threadIdx.x] = vtc[j + k] >> 2;
+            stage_quads(buf, 0, vtc + i, na, 2);
+            stage_quads(buf, na, vtc + j, nb, 2);
             i = 0, ie = na, j = na, je = na + nb;
             int32_t x = i < ie ? buf[i][threadIdx.x] : INF, y = j < je ? buf[j][threadIdx.x] : INF;
             const int32_t last = max(je - 1, 0);
             while (x != INF && y != INF) {  // branch-free advance (two reads per step)
-                if (x == y) {
-                    row[n] = x;
-                    n++;
-                }
+                if (x == y) o.put(x);
                 const int32_t m = min(x, y);
                 i += x == m;
                 j += y == m;
@@ -352,16 +425,14 @@ __global__ __launch_bounds__(ET_THREADS) void k_et(const int32_t *__restrict__ e
         } else {
             int32_t x = i < ie ? vtc[i] >> 2 : INF, y = j < je ? vtc[j] >> 2 : INF;
             while (x != INF && y != INF) {
-                if (x == y) {
-                    row[n] = x;
-                    n++;
-                }
+                if (x == y) o.put(x);
                 const int32_t m = min(x, y);
                 if (x == m) x = ++i < ie ? vtc[i] >> 2 : INF;
                 if (y == m) y = ++j < je ? vtc[j] >> 2 : INF;
             }
         }
-        cnt[e] = n;
+        o.flush();
+        cnt[e] = o.n;
     }
 }
 
@@ -686,64 +757,9 @@ __global__ void k_ef_copy(const int64_t *__restrict__ et_off, int64_t E,
     }
 }
 
-// Merges of a few short sorted rows (TT, EE) stage the rows in LDS first:
-// one column of ROW_CAP words per thread (consecutive threads in
-// consecutive banks), filled by independent loads, so a row costs one
-// memory latency instead of one per element; the merge then reads LDS.
-// A thread whose rows exceed ROW_CAP merges from global memory.
+// TT and EE merge columns (stage_quads): ROW_CAP words per thread.
 constexpr int ROW_THREADS = 64;
 constexpr int ROW_CAP = 112;  // 28 KB per 64 threads: LDS bounds the resident waves
-
-__device__ __forceinline__ void stage_row(int32_t (*buf)[ROW_THREADS], int at,
-                                          const int32_t *__restrict__ src, int32_t n, int shift) {
-#pragma unroll 8
-    for (int32_t k = 0; k < n; k++) buf[at + k][threadIdx.x] = src[k] >> shift;
-}
-
-// Unaligned 16- and 8-byte loads: dword alignment is enough for
-// global_load_dwordx4/x2 on gfx950.
-typedef int32_t i32x4u __attribute__((ext_vector_type(4), aligned(4)));
-typedef int32_t i32x2u __attribute__((ext_vector_type(2), aligned(4)));
-
-// stage_row by 16-byte loads.  Whole quads are read, up to three words past
-// the row (the source array carries that slack), and all four words go to
-// the column: the words past the row land where the next row is staged, or
-// in the column's three spare slots after the last row.
-__device__ __forceinline__ void stage_quads(int32_t (*buf)[ROW_THREADS], int at,
-                                            const int32_t *__restrict__ src, int32_t n,
-                                            int shift) {
-#pragma unroll 4
-    for (int32_t k = 0; k < n; k += 4) {
-        const i32x4u v = *reinterpret_cast<const i32x4u *>(src + k);
-        buf[at + k][threadIdx.x] = v.x >> shift;
-        buf[at + k + 1][threadIdx.x] = v.y >> shift;
-        buf[at + k + 2][threadIdx.x] = v.z >> shift;
-        buf[at + k + 3][threadIdx.x] = v.w >> shift;
-    }
-}
-
-// A thread's output row written in 16-byte stores: words gather in a
-// register quad, stored when full; the last, partial quad is stored whole,
-// so the row's slot must start 16-byte aligned and span a multiple of four
-// words.  One store per four words instead of one per word: scattered
-// per-thread rows cost the vector memory path per store instruction.
-struct QuadOut {
-    int32_t *row;
-    int64_t n = 0;
-    int4 q = make_int4(0, 0, 0, 0);
-    __device__ __forceinline__ void put(int32_t v) {
-        const int k = (int)(n & 3);
-        q.x = k == 0 ? v : q.x;
-        q.y = k == 1 ? v : q.y;
-        q.z = k == 2 ? v : q.z;
-        q.w = k == 3 ? v : q.w;
-        if (k == 3) *reinterpret_cast<int4 *>(row + n - 3) = q;
-        n++;
-    }
-    __device__ __forceinline__ void flush() {
-        if (n & 3) *reinterpret_cast<int4 *>(row + (n & ~(int64_t)3)) = q;
-    }
-};
 
 // TT(t): union of VT rows of t's corners, minus t (4-way merge), written in
 // one pass to a padded slot at pad[t] of |VT rows| - 4 words rounded up to a
@@ -958,17 +974,17 @@ __global__ __launch_bounds__(ROW_THREADS) void k_ee(const int32_t *__restrict__ 
                                                     const int32_t *__restrict__ eids,
                                                     const int64_t *__restrict__ off,
                                                     int32_t *__restrict__ out) {
-    __shared__ int32_t buf[ROW_CAP][ROW_THREADS];
+    __shared__ int32_t buf[ROW_CAP + 3][ROW_THREADS];  // three spare slots for stage_quads
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
          e += (int64_t)gridDim.x * blockDim.x) {
         const int32_t a = ev[2 * e], b = ev[2 * e + 1];
         int32_t i = vv_off[a], ie = vv_off[a + 1], j = vv_off[b], je = vv_off[b + 1];
-        int32_t *row = out + off[e];
-        int64_t m = 0;
+        QuadOutExact o;
+        o.row = out + off[e];
         const int32_t na = ie - i, nb = je - j;
         if (na + nb <= ROW_CAP) {
-            stage_row(buf, 0, eids + i, na, 0);
-            stage_row(buf, na, eids + j, nb, 0);
+            stage_quads(buf, 0, eids + i, na, 0);
+            stage_quads(buf, na, eids + j, nb, 0);
             i = 0, ie = na, j = na, je = na + nb;
             int32_t x = i < ie ? buf[i][threadIdx.x] : INF, y = j < je ? buf[j][threadIdx.x] : INF;
             const int32_t last = max(je - 1, 0);
@@ -979,7 +995,7 @@ __global__ __launch_bounds__(ROW_THREADS) void k_ee(const int32_t *__restrict__ 
                 const int32_t xa = buf[min(i, last)][threadIdx.x], yb = buf[min(j, last)][threadIdx.x];
                 x = i < ie ? xa : INF;
                 y = j < je ? yb : INF;
-                if (mn != (int32_t)e) row[m++] = mn;
+                if (mn != (int32_t)e) o.put(mn);
             }
         } else {
             int32_t x = i < ie ? eids[i] : INF, y = j < je ? eids[j] : INF;
@@ -987,9 +1003,10 @@ __global__ __launch_bounds__(ROW_THREADS) void k_ee(const int32_t *__restrict__ 
                 const int32_t mn = min(x, y);
                 if (x == mn) x = ++i < ie ? eids[i] : INF;
                 if (y == mn) y = ++j < je ? eids[j] : INF;
-                if (mn != (int32_t)e) row[m++] = mn;
+                if (mn != (int32_t)e) o.put(mn);
             }
         }
+        o.flush();
     }
 }
 
@@ -1372,7 +1389,8 @@ int ensure_ee(tg_mesh &m) {
     if (E > 0) {
         // each vertex's edge ids (the vv_eid row) sorted ascending
         const int64_t n2 = 2 * E;
-        if ((rc = m.tmp_c.ensure(sizeof(int32_t) * (size_t)n2))) return rc;
+        // four words of slack: k_ee stages rows by whole 16-byte quads
+        if ((rc = m.tmp_c.ensure(sizeof(int32_t) * (size_t)(n2 + 4)))) return rc;
         TG_HIP(hipMemcpyAsync(m.tmp_c.p, m.vv_eid.p, sizeof(int32_t) * (size_t)n2,
                               hipMemcpyDeviceToDevice, m.stream));
         if ((rc = launch_sort_rows_i32(m.vv_off.as<int32_t>(), m.tmp_c.as<int32_t>(), V,
