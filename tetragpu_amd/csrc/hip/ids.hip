@@ -759,7 +759,7 @@ __global__ void k_ef_copy(const int64_t *__restrict__ et_off, int64_t E,
 
 // TT and EE merge columns (stage_quads): ROW_CAP words per thread.
 constexpr int ROW_THREADS = 64;
-constexpr int ROW_CAP = 112;  // 28 KB per 64 threads: LDS bounds the resident waves
+constexpr int ROW_CAP = 100;  // 26 KB per 64 threads (with three spare slots): six waves per CU
 
 // TT(t): union of VT rows of t's corners, minus t (4-way merge), written in
 // one pass to a padded slot at pad[t] of |VT rows| - 4 words rounded up to a
@@ -880,8 +880,9 @@ __global__ __launch_bounds__(256) void k_pack_rows(const int64_t *__restrict__ p
 
 // k_pack_rows for long rows (tens of words on average): one wave per 64
 // consecutive rows, the lanes copying a row 64 words at a time; the first
-// 64 words of four rows are loaded together.  Row bounds are read once per
-// wave, one row per lane, and broadcast with readlane.
+// 64 words of PACK_GROUP rows are loaded together.  Row bounds are read once
+// per wave, one row per lane, and broadcast with readlane.
+constexpr int PACK_GROUP = 16;
 __device__ __forceinline__ int64_t readlane64(int64_t v, int lane) {
     const int32_t lo = __builtin_amdgcn_readlane((int32_t)(uint32_t)v, lane);
     const int32_t hi = __builtin_amdgcn_readlane((int32_t)(v >> 32), lane);
@@ -900,11 +901,11 @@ __global__ __launch_bounds__(256) void k_pack_long_rows(const int64_t *__restric
     const int64_t mp = lane < nr ? pad[r0 + lane] : 0;
     const int64_t mo = lane < nr ? off[r0 + lane] : 0;
     const int32_t ml = lane < nr ? (int32_t)(off[r0 + lane + 1] - mo) : 0;
-    for (int i = 0; i < nr; i += 4) {
-        int64_t p[4], o[4];
-        int32_t n[4], v[4];
+    for (int i = 0; i < nr; i += PACK_GROUP) {
+        int64_t p[PACK_GROUP], o[PACK_GROUP];
+        int32_t n[PACK_GROUP], v[PACK_GROUP];
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
+        for (int u = 0; u < PACK_GROUP; u++) {
             const int r = min(i + u, nr - 1);
             p[u] = readlane64(mp, r);
             o[u] = readlane64(mo, r);
@@ -913,7 +914,7 @@ __global__ __launch_bounds__(256) void k_pack_long_rows(const int64_t *__restric
             v[u] = src[n[u] > 0 ? p[u] + min(lane, n[u] - 1) : 0];
         }
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
+        for (int u = 0; u < PACK_GROUP; u++) {
             if (lane < n[u]) dst[o[u] + lane] = v[u];
             for (int32_t k = 64 + lane; k < n[u]; k += 64) dst[o[u] + k] = src[p[u] + k];
         }
