@@ -158,7 +158,7 @@ __global__ __launch_bounds__(EDGE_THREADS) void k_edge_first(
     const int32_t *__restrict__ vtc, const int32_t *__restrict__ vv_off,
     const int32_t *__restrict__ vv, int64_t V, int32_t *__restrict__ efirst,
     uint8_t *__restrict__ flags, int32_t *__restrict__ te_idx) {
-    __shared__ int32_t srow[EDGE_CAP][EDGE_THREADS];
+    __shared__ int32_t srow[EDGE_CAP + 3][EDGE_THREADS];  // three spare slots for stage_quads
     __shared__ int32_t sef[EDGE_CAP][EDGE_THREADS];
     const int tid = threadIdx.x;
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + tid; v < V;
@@ -168,11 +168,8 @@ __global__ __launch_bounds__(EDGE_THREADS) void k_edge_first(
         int32_t *ef = efirst + rb;
         const int32_t kb = vt_off[v], ke = vt_off[v + 1];
         if (rn <= EDGE_CAP) {
-#pragma unroll 8
-            for (int32_t i = 0; i < rn; i++) {
-                srow[i][tid] = row[i];
-                sef[i][tid] = INF;
-            }
+            stage_quads(srow, 0, row, rn, 0);
+            for (int32_t i = 0; i < rn; i++) sef[i][tid] = INF;
             for (int32_t k0 = kb; k0 < ke; k0 += 8) {
                 int32_t pc[8];
                 int4 q[8];
@@ -200,11 +197,14 @@ __global__ __launch_bounds__(EDGE_THREADS) void k_edge_first(
                     }
                 }
             }
+            QuadOutExact o;
+            o.row = ef;
             for (int32_t i = 0; i < rn; i++) {
                 const int32_t e = sef[i][tid];
-                ef[i] = e;
+                o.put(e);
                 if (srow[i][tid] > v && e != INF) flags[e] = 1;
             }
+            o.flush();
             continue;
         }
         for (int32_t k = kb; k < ke; k++) {

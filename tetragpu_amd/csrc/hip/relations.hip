@@ -101,7 +101,7 @@ int prepare_hot_path(tg_mesh &m) {
     const size_t V = (size_t)m.n_points, T = (size_t)m.n_cells, I = sizeof(int32_t);
     int rc;
     if ((rc = m.scratch.ensure(I * 12 * T)) || (rc = m.deg.ensure(I * V)) ||
-        (rc = m.vv_off.ensure(I * (V + 1))) || (rc = m.vv_cols.ensure(I * 12 * T)) ||
+        (rc = m.vv_off.ensure(I * (V + 1))) || (rc = m.vv_cols.ensure(I * (12 * T + 4))) ||
         (rc = m.classes.ensure(I * 3 * V)) || (rc = m.heavy.ensure(I * V)) ||
         (rc = m.fix.ensure(I * V)) || (rc = m.fixbig.ensure(I * V)) ||
         (rc = m.counters.ensure(I * N_COUNTERS)) ||
