@@ -95,9 +95,22 @@ def test_kuhn_permuted_unsorted(gpu, order):
     check_all(gpu, m.cells, m.n_points)
 
 
-def test_kuhn_hubs(gpu):
-    m = gpu.kuhn(6, seed=11, split_fraction=0.2, n_hubs=4, hub_splits=40)
-    check_all(gpu, m.cells, m.n_points)
+@pytest.mark.parametrize("hub_splits", [40, 90])
+def test_kuhn_hubs(gpu, hub_splits):
+    """High-valence hub vertices: rows beyond the LDS merge columns (100 words
+    for TT and EE, 125 for ET, 48 for the edge walk) merge from global memory,
+    and TT rows beyond 128 words take several chunks of the row pack."""
+    m = gpu.kuhn(6, seed=11, split_fraction=0.2, n_hubs=4, hub_splits=hub_splits)
+    ref = check_all(gpu, m.cells, m.n_points)
+    vt_len = np.diff(ref["VT"].off)
+    ev = np.asarray(ref["EV"]).reshape(-1, 2)
+    deg = np.bincount(ev.ravel(), minlength=m.n_points)
+    assert vt_len[m.cells].sum(axis=1).max() > 100
+    assert np.diff(ref["TT"].off).max() > 128
+    assert (vt_len[ev[:, 0]] + vt_len[ev[:, 1]]).max() > 125
+    assert deg.max() > 48
+    if hub_splits == 90:
+        assert (deg[ev[:, 0]] + deg[ev[:, 1]]).max() > 100
 
 
 @pytest.mark.parametrize("seed,V,T", [(1, 30, 40), (2, 200, 300), (3, 60, 400), (5, 400, 700)])

@@ -739,8 +739,12 @@ __global__ __launch_bounds__(ET_THREADS) void k_ef(const int64_t *__restrict__ e
                 }
             }
         }
-        if (lds)
-            for (int64_t q = 0; q < m; q++) grow[q] = col[q][threadIdx.x];
+        if (lds) {
+            QuadOutExact o;
+            o.row = grow;
+            for (int64_t q = 0; q < m; q++) o.put(col[q][threadIdx.x]);
+            o.flush();
+        }
         cnt[e] = m;
     }
 }
@@ -751,9 +755,20 @@ __global__ void k_ef_copy(const int64_t *__restrict__ et_off, int64_t E,
                           int32_t *__restrict__ out) {
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
          e += (int64_t)gridDim.x * blockDim.x) {
+        // 16-byte loads (the padded buffer carries four words of slack) and
+        // stores, the last one to three words one by one
         const int32_t *src = tmp + 2 * et_off[e];
         const int64_t o = off[e], n = off[e + 1] - o;
-        for (int64_t q = 0; q < n; q++) out[o + q] = src[q];
+        QuadOutExact w;
+        w.row = out + o;
+        for (int64_t q = 0; q < n; q += 4) {
+            const i32x4u v = *reinterpret_cast<const i32x4u *>(src + q);
+            w.put(v.x);
+            if (q + 1 < n) w.put(v.y);
+            if (q + 2 < n) w.put(v.z);
+            if (q + 3 < n) w.put(v.w);
+        }
+        w.flush();
     }
 }
 
@@ -1287,7 +1302,7 @@ int ensure_ef(tg_mesh &m) {
     int64_t et_nnz = 0;
     if ((rc = read_i64(m, m.et_off.p, E, et_nnz))) return rc;
     if ((rc = m.ef_off.ensure(sizeof(int64_t) * (size_t)(E + 1))) ||
-        (rc = m.tmp_c.ensure(sizeof(int32_t) * 2 * (size_t)std::max<int64_t>(et_nnz, 1))))
+        (rc = m.tmp_c.ensure(sizeof(int32_t) * (2 * (size_t)std::max<int64_t>(et_nnz, 1) + 4))))
         return rc;
     int64_t *off = m.ef_off.as<int64_t>();
     if (E > 0) {
