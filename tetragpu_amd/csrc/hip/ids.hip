@@ -328,10 +328,9 @@ __global__ __launch_bounds__(VE_THREADS) void k_ve_fill(const int32_t *__restric
                 }
                 col[p][threadIdx.x] = key;
             }
-            for (int32_t k = 0; k < n; k++) {
+            for (int32_t k = 0; k < n; k++) {  // (hi, id) pairs, one 8-byte store each
                 const int64_t key = col[k][threadIdx.x];
-                row[2 * k] = (int32_t)(uint32_t)key;
-                row[2 * k + 1] = (int32_t)(key >> 32);
+                reinterpret_cast<int2 *>(row)[k] = make_int2((int32_t)(uint32_t)key, (int32_t)(key >> 32));
             }
         } else {
             for (int32_t k = 0; k < n; k++) {
@@ -617,9 +616,9 @@ __global__ void k_vf(const int4 *__restrict__ tv4, const int32_t *__restrict__ v
                     if (FILL) {
                         int lo, mid, hi;
                         face_pos(sl, lo, mid, hi);
-                        row[3 * n] = corner(q[c], mid);
-                        row[3 * n + 1] = corner(q[c], hi);
-                        row[3 * n + 2] = id[c][m];
+                        // one 12-byte store per entry
+                        const int32_t ent[3] = {corner(q[c], mid), corner(q[c], hi), id[c][m]};
+                        __builtin_memcpy(row + 3 * n, ent, sizeof ent);
                     }
                     n++;
                 }
