@@ -892,46 +892,42 @@ __global__ __launch_bounds__(256) void k_pack_rows(const int64_t *__restrict__ p
     }
 }
 
-// k_pack_rows for long rows (tens of words on average): one wave per 64
-// consecutive rows, the lanes copying a row 64 words at a time; the first
-// 64 words of PACK_GROUP rows are loaded together.  Row bounds are read once
-// per wave, one row per lane, and broadcast with readlane.
-constexpr int PACK_GROUP = 16;
-__device__ __forceinline__ int64_t readlane64(int64_t v, int lane) {
-    const int32_t lo = __builtin_amdgcn_readlane((int32_t)(uint32_t)v, lane);
-    const int32_t hi = __builtin_amdgcn_readlane((int32_t)(v >> 32), lane);
-    return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
-}
-
-__global__ __launch_bounds__(256) void k_pack_long_rows(const int64_t *__restrict__ pad,
+// Pack of long padded rows with a lane per output word: one wave per 64
+// consecutive rows copies their contiguous output range 64 words at a time
+// (full, aligned stores; reads nearly contiguous, the slots being
+// consecutive), each word finding its row by a six-step search of the
+// rows' output starts held in LDS.  Four chunks per lane are in flight.
+__global__ __launch_bounds__(256) void k_pack_rows_wave(const int64_t *__restrict__ pad,
                                                         const int64_t *__restrict__ off,
                                                         int64_t rows,
                                                         const int32_t *__restrict__ src,
                                                         int32_t *__restrict__ dst) {
-    const int lane = threadIdx.x & 63;
-    const int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
-    if (r0 >= rows) return;  // the whole wave
-    const int nr = (int)min<int64_t>(64, rows - r0);
-    const int64_t mp = lane < nr ? pad[r0 + lane] : 0;
-    const int64_t mo = lane < nr ? off[r0 + lane] : 0;
-    const int32_t ml = lane < nr ? (int32_t)(off[r0 + lane + 1] - mo) : 0;
-    for (int i = 0; i < nr; i += PACK_GROUP) {
-        int64_t p[PACK_GROUP], o[PACK_GROUP];
-        int32_t n[PACK_GROUP], v[PACK_GROUP];
+    __shared__ int32_t s_start[4][65];  // output start of each row, relative to the wave's
+    __shared__ int64_t s_pad[4][64];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t r0 = ((int64_t)blockIdx.x * 4 + w) * 64;
+    const int nr = r0 < rows ? (int)min<int64_t>(64, rows - r0) : 0;
+    const int64_t o0 = nr > 0 ? off[r0] : 0;
+    // rows past the end start at INT32_MAX and are never chosen
+    s_start[w][lane] = lane < nr ? (int32_t)(off[r0 + lane] - o0) : INT32_MAX;
+    if (lane == 0) s_start[w][64] = nr > 0 ? (int32_t)(off[r0 + nr] - o0) : 0;
+    s_pad[w][lane] = lane < nr ? pad[r0 + lane] : 0;
+    __syncthreads();
+    const int32_t total = s_start[w][64];
+    for (int32_t k0 = 0; k0 < total; k0 += 4 * 64) {
+        int32_t v[4];
 #pragma unroll
-        for (int u = 0; u < PACK_GROUP; u++) {
-            const int r = min(i + u, nr - 1);
-            p[u] = readlane64(mp, r);
-            o[u] = readlane64(mo, r);
-            n[u] = i + u < nr ? __builtin_amdgcn_readlane(ml, r) : 0;
-            // clamped to the row (or to word 0 for an empty row): every load is in bounds
-            v[u] = src[n[u] > 0 ? p[u] + min(lane, n[u] - 1) : 0];
+        for (int u = 0; u < 4; u++) {
+            const int32_t k = min(k0 + 64 * u + lane, total - 1);
+            int lo = 0;
+#pragma unroll
+            for (int step = 32; step >= 1; step >>= 1)
+                if (s_start[w][lo + step] <= k) lo += step;
+            v[u] = src[s_pad[w][lo] + (k - s_start[w][lo])];
         }
 #pragma unroll
-        for (int u = 0; u < PACK_GROUP; u++) {
-            if (lane < n[u]) dst[o[u] + lane] = v[u];
-            for (int32_t k = 64 + lane; k < n[u]; k += 64) dst[o[u] + k] = src[p[u] + k];
-        }
+        for (int u = 0; u < 4; u++)
+            if (k0 + 64 * u + lane < total) dst[o0 + k0 + 64 * u + lane] = v[u];
     }
 }
 
@@ -1351,10 +1347,10 @@ int ensure_tt(tg_mesh &m) {
     if ((rc = scan_counts64(m, off, T, nnz))) return rc;
     if ((rc = m.tt_vals.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)))) return rc;
     if (T > 0) {
-        hipLaunchKernelGGL(k_pack_long_rows, dim3((unsigned)((T + 255) / 256)), dim3(256), 0,
+        hipLaunchKernelGGL(k_pack_rows_wave, dim3((unsigned)((T + 255) / 256)), dim3(256), 0,
                            m.stream, pad, off, T, m.tmp_c.as<int32_t>(),
                            m.tt_vals.as<int32_t>());
-        TG_LAUNCH_CHECK("k_pack_long_rows");
+        TG_LAUNCH_CHECK("k_pack_rows_wave");
     }
     m.tt_nnz = nnz;
     m.tt_built = true;
