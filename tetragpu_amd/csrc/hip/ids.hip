@@ -775,12 +775,67 @@ __global__ void k_ef_copy(const int64_t *__restrict__ et_off, int64_t E,
 constexpr int ROW_THREADS = 64;
 constexpr int ROW_CAP = 100;  // 26 KB per 64 threads (with three spare slots): six waves per CU
 
+// Cells in the order of their corner-0 entries in VT (rows by vertex, cells
+// ascending within a row), for k_tt on meshes with rows beyond its LDS
+// column: neighbouring lanes then take cells around one vertex and share
+// its VT row, where the mesh's own cell order may scatter them (on a mesh
+// of uniform short rows the pass costs more than the sharing saves).  One pass counts the corner-0 codes per chunk, a scan of
+// the chunk counts gives each chunk's start, a second pass fills the order
+// stably (block-wide ballot scans, one per round of 256 codes).
+constexpr int C0_THREADS = 256, C0_PER = 16, C0_CHUNK = C0_THREADS * C0_PER;
+
+__global__ __launch_bounds__(C0_THREADS) void k_c0_count(const int32_t *__restrict__ vtc,
+                                                         int64_t n, int64_t *__restrict__ bcnt) {
+    __shared__ int32_t total;
+    if (threadIdx.x == 0) total = 0;
+    __syncthreads();
+    const int64_t b0 = (int64_t)blockIdx.x * C0_CHUNK;
+    int32_t c = 0;
+    for (int i = 0; i < C0_PER; i++) {
+        const int64_t k = b0 + (int64_t)i * C0_THREADS + threadIdx.x;
+        c += k < n && (vtc[k] & 3) == 0;
+    }
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&total, c);
+    __syncthreads();
+    if (threadIdx.x == 0) bcnt[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(C0_THREADS) void k_c0_fill(const int32_t *__restrict__ vtc, int64_t n,
+                                                        const int64_t *__restrict__ bbase,
+                                                        int32_t *__restrict__ order) {
+    __shared__ int32_t wsum[C0_THREADS / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t b0 = (int64_t)blockIdx.x * C0_CHUNK;
+    int64_t base = bbase[blockIdx.x];
+    for (int i = 0; i < C0_PER; i++) {
+        const int64_t k = b0 + (int64_t)i * C0_THREADS + threadIdx.x;
+        const int32_t code = k < n ? vtc[k] : 1;
+        const bool f = (code & 3) == 0;
+        const unsigned long long m = __ballot(f);
+        const int before = __popcll(m & ((1ull << lane) - 1));
+        if (lane == 0) wsum[w] = __popcll(m);
+        __syncthreads();
+        int32_t off = 0, tot = 0;
+#pragma unroll
+        for (int j = 0; j < C0_THREADS / 64; j++) {
+            const int32_t x = wsum[j];
+            off += j < w ? x : 0;
+            tot += x;
+        }
+        if (f) order[base + off + before] = code >> 2;
+        base += tot;
+        __syncthreads();  // wsum is rewritten next round
+    }
+}
+
 // TT(t): union of VT rows of t's corners, minus t (4-way merge), written in
 // one pass to a padded slot at pad[t] of |VT rows| - 4 words rounded up to a
 // multiple of four, with its length in cnt[t]; the rows are packed after
 // the scan of cnt.
 __global__ void k_tt_bound(const int4 *__restrict__ tv4, int64_t T,
-                           const int32_t *__restrict__ vt_off, int64_t *__restrict__ ub) {
+                           const int32_t *__restrict__ vt_off, int64_t *__restrict__ ub,
+                           int32_t *__restrict__ long_rows) {
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < T;
          t += (int64_t)gridDim.x * blockDim.x) {
         const int4 q = tv4[t];
@@ -790,18 +845,21 @@ __global__ void k_tt_bound(const int4 *__restrict__ tv4, int64_t T,
         const i32x2u r3 = *reinterpret_cast<const i32x2u *>(vt_off + q.w);
         const int64_t n = (int64_t)(r0.y - r0.x) + (r1.y - r1.x) + (r2.y - r2.x) + (r3.y - r3.x) - 4;
         ub[t] = (n + 3) & ~(int64_t)3;
+        if (n + 4 > ROW_CAP) *long_rows = 1;  // every writer stores the same value
     }
 }
 
 __global__ __launch_bounds__(ROW_THREADS) void k_tt(const int4 *__restrict__ tv4, int64_t T,
                                                     const int32_t *__restrict__ vt_off,
                                                     const int32_t *__restrict__ vtc,
+                                                    const int32_t *__restrict__ order,
                                                     const int64_t *__restrict__ pad,
                                                     int64_t *__restrict__ cnt,
                                                     int32_t *__restrict__ out) {
     __shared__ int32_t buf[ROW_CAP + 3][ROW_THREADS];  // three spare slots for stage_quads
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < T;
-         t += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < T;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t t = order ? order[i] : i;  // cells in VT order when given
         const int4 q = tv4[t];
         const i32x2u r0 = *reinterpret_cast<const i32x2u *>(vt_off + q.x);
         const i32x2u r1 = *reinterpret_cast<const i32x2u *>(vt_off + q.y);
@@ -1326,21 +1384,46 @@ int ensure_tt(tg_mesh &m) {
     if ((rc = ensure_vt_sorted(m))) return rc;
     const int64_t T = m.n_cells;
     if ((rc = m.tt_off.ensure(sizeof(int64_t) * (size_t)(T + 1))) ||
-        (rc = m.tmp_b.ensure(sizeof(int64_t) * (size_t)(T + 1))))
+        (rc = m.tmp_b.ensure(sizeof(int64_t) * (size_t)(T + 2))))
         return rc;
     int64_t *off = m.tt_off.as<int64_t>(), *pad = m.tmp_b.as<int64_t>();
+    // pad[T + 1]: set when a cell's rows exceed the LDS column (they merge
+    // from global memory, where lanes that share rows gain most)
+    int32_t *long_rows = reinterpret_cast<int32_t *>(pad + T + 1);
     if (T > 0) {
+        hipLaunchKernelGGL(k_fill_i32v, dim3(1), dim3(64), 0, m.stream, long_rows, 0, (int64_t)2);
         hipLaunchKernelGGL(k_tt_bound, dim3(grid(T)), dim3(256), 0, m.stream,
-                           m.tv.as<const int4>(), T, m.vt_off.as<int32_t>(), pad);
+                           m.tv.as<const int4>(), T, m.vt_off.as<int32_t>(), pad, long_rows);
         TG_LAUNCH_CHECK("k_tt_bound");
     }
-    int64_t padded = 0;
+    int64_t padded = 0, has_long = 0;
     if ((rc = scan_counts64(m, pad, T, padded))) return rc;
-    if ((rc = m.tmp_c.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(padded, 1)))) return rc;
+    if (T > 0 && (rc = read_i64(m, pad, T + 1, has_long))) return rc;
+    // tmp_c: the padded rows, then the cell order and its chunk counts
+    const int64_t nchunk = (4 * T + C0_CHUNK - 1) / C0_CHUNK;
+    const int64_t at_order = (padded + 1) & ~(int64_t)1;  // 8-byte aligned chunk counts
+    const int64_t at_cnt = at_order + ((T + 1) & ~(int64_t)1);
+    if ((rc = m.tmp_c.ensure(sizeof(int32_t) * (size_t)(at_cnt + 2 * (nchunk + 1))))) return rc;
+    const int32_t *order = nullptr;
+    if (T > 0 && has_long) {
+        int32_t *ord = m.tmp_c.as<int32_t>() + at_order;
+        int64_t *bcnt = reinterpret_cast<int64_t *>(m.tmp_c.as<int32_t>() + at_cnt);
+        hipLaunchKernelGGL(k_c0_count, dim3((unsigned)nchunk), dim3(C0_THREADS), 0, m.stream,
+                           m.vtc.as<int32_t>(), 4 * T, bcnt);
+        TG_LAUNCH_CHECK("k_c0_count");
+        int64_t found = 0;
+        if ((rc = scan_counts64(m, bcnt, nchunk, found))) return rc;
+        if (found == T) {  // one corner-0 code per cell
+            hipLaunchKernelGGL(k_c0_fill, dim3((unsigned)nchunk), dim3(C0_THREADS), 0, m.stream,
+                               m.vtc.as<int32_t>(), 4 * T, bcnt, ord);
+            TG_LAUNCH_CHECK("k_c0_fill");
+            order = ord;
+        }
+    }
     if (T > 0) {
         hipLaunchKernelGGL(k_tt, dim3(grid(T, ROW_THREADS)), dim3(ROW_THREADS), 0, m.stream,
                            m.tv.as<const int4>(), T, m.vt_off.as<int32_t>(), m.vtc.as<int32_t>(),
-                           pad, off, m.tmp_c.as<int32_t>());
+                           order, pad, off, m.tmp_c.as<int32_t>());
         TG_LAUNCH_CHECK("k_tt");
     }
     int64_t nnz = 0;
