@@ -1,0 +1,132 @@
+"""Runs one GPU test case on the CPU kernel emulator.
+
+Started by tests/test_emu_kernels.py as a child process with the
+AddressSanitizer runtime preloaded; the package's ctypes layer is pointed at
+tests/emu/lib/libtetra_hip_emu.so for this process only.  Prints "ok <case>"
+and exits 0 when the case's results equal the oracle's; an out-of-bounds
+access aborts the process with ASan's report.
+
+Usage: python tests/emu/run_case.py <case> [args...]
+"""
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parent.parent
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "tests"))
+
+import numpy as np  # noqa: E402
+
+import tetragpu_amd as t  # noqa: E402
+from tetragpu_amd import _native  # noqa: E402
+
+
+def use_emulator() -> None:
+    import os
+
+    path = os.environ.get("TETRA_EMU_LIB", str(HERE / "lib" / "libtetra_hip_emu.so"))
+    lib = _native.C.CDLL(path, mode=_native.C.RTLD_GLOBAL)
+    for fn, (res, args) in _native._HIP_SIGS.items():
+        f = getattr(lib, fn, None)  # an older build may lack newer entry points
+        if f is not None:
+            f.restype, f.argtypes = res, args
+    _native._hip = lib
+
+
+def case_after_hot_path(seed: int = 8, n: int = 5, variant: int = 1) -> None:
+    """TE, then the hot path, then ET/TT/VF on one mesh."""
+    from test_gpu_relations import oracle_all, same_csr, same_fixed
+
+    m = t.kuhn(n, seed=seed)
+    ref = oracle_all(m.cells, m.n_points)
+    with t.Mesh(m.cells, m.n_points, m.scalars) as d:
+        d.set_config(variant=variant)
+        d.build(t.flags("TE"))
+        d.vv_critpoints()
+        d.build(t.flags("ET", "TT", "VF"))
+        same_csr(d.relation("ET"), ref["ET"], "ET")
+        same_csr(d.relation("TT"), ref["TT"], "TT")
+        same_csr(d.relation("VF"), ref["VF"], "VF")
+        same_fixed(d.relation("TE"), ref["TE"], "TE")
+
+
+def case_all_relations(seed: int = 3, n: int = 4, variant: int = 0) -> None:
+    """Every relation flag in one build, against the oracle."""
+    from test_gpu_relations import check_all
+
+    m = t.kuhn(n, seed=seed)
+    check_all(t, m.cells, m.n_points, variant)
+
+
+def case_vv_cp(seed: int = 5, n: int = 6, variant: int = 2) -> None:
+    """VV + classes in one kernel form, against the oracle."""
+    from oracle import binding as O
+
+    m = t.kuhn(n, seed=seed)
+    with t.Mesh(m.cells, m.n_points, m.scalars) as d:
+        d.set_config(variant=variant)
+        d.vv_critpoints()
+        off, vals = d.relation("VV")
+        cls = d.classes()
+    VV = O.vv(m.cells, m.n_points)
+    S = VV.sorted_rows()
+    assert np.array_equal(off, S.off) and np.array_equal(vals[:, 0], S.val[:, 0]), "VV"
+    ref = O.critpoints(VV, m.scalars.astype(np.float64), m.n_points)
+    assert np.array_equal(cls, ref), "classes"
+
+
+def case_hubs(seed: int = 11, hub_splits: int = 40) -> None:
+    """High-valence hubs: rows beyond the LDS columns take the global paths."""
+    from test_gpu_relations import check_all
+
+    m = t.kuhn(4, seed=seed, split_fraction=0.2, n_hubs=3, hub_splits=hub_splits)
+    check_all(t, m.cells, m.n_points)
+
+
+def case_soup(seed: int = 2, V: int = 60, T: int = 200) -> None:
+    """A random cell soup (non-manifold, faces over more than two cells)."""
+    from test_gpu_relations import check_all
+
+    rng = np.random.default_rng(seed)
+    cells = np.array([rng.choice(V, 4, replace=False) for _ in range(T)], np.int32)
+    check_all(t, cells, V, expect_ft=False)
+
+
+def case_routes(seed: int = 4, route: int = 2, bv: int = 7) -> None:
+    """VV + classes and VT through one routing of the bucketed form."""
+    from oracle import binding as O
+
+    m = t.kuhn(7, seed=seed)
+    with t.Mesh(m.cells, m.n_points, m.scalars) as d:
+        d.set_config(variant=2)
+        d.set_bucket(bv, 4096)
+        d.set_routing(route)
+        d.vv_critpoints()
+        d.build(t.flags("VT"))
+        cls = d.classes()
+        vt = d.relation("VT")
+    VV = O.vv(m.cells, m.n_points)
+    assert np.array_equal(cls, O.critpoints(VV, m.scalars.astype(np.float64), m.n_points))
+    ref = O.vt(m.cells, m.n_points)
+    assert np.array_equal(vt[0], ref.off) and np.array_equal(vt[1], ref.val), "VT"
+
+
+CASES = {"after_hot_path": case_after_hot_path, "all_relations": case_all_relations,
+         "vv_cp": case_vv_cp, "hubs": case_hubs, "soup": case_soup, "routes": case_routes}
+
+
+def main() -> int:
+    name = sys.argv[1]
+    args = [int(a) for a in sys.argv[2:]]
+    use_emulator()
+    t.init(0)
+    CASES[name](*args)
+    print(f"ok {name} {' '.join(sys.argv[2:])}", flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -94,5 +94,8 @@ int launch_cp_fix(VvCpBuffers &b, hipStream_t s);
 // Whole pipeline: counters reset, VT corners, VV, critical points.
 int run_vv_cp(VvCpBuffers &b, const VvCpConfig &cfg, bool classify,
               hipStream_t s);
+// The same on corner lists that are already built (vt_off / vtc untouched).
+int run_vv_cp_on_corners(VvCpBuffers &b, const VvCpConfig &cfg, bool classify,
+                         hipStream_t s);
 
 }  // namespace tg

@@ -227,9 +227,16 @@ static int run_vv(tg_mesh &m, bool classify) {
     } else {
         if ((rc = prepare_vt(m))) return rc;
         VvCpBuffers b = hot_buffers(m);
-        if ((rc = run_vv_cp(b, m.cfg, classify, m.stream))) return rc;
-        m.vt_corners_built = true;
-        m.vt_sorted = false;
+        if (m.vt_corners_built) {
+            // the corner lists are already there (rows in any order serve the
+            // lane kernel): keep them as they are, sorted rows included, so
+            // relations built before and after this run read one state
+            if ((rc = run_vv_cp_on_corners(b, m.cfg, classify, m.stream))) return rc;
+        } else {
+            if ((rc = run_vv_cp(b, m.cfg, classify, m.stream))) return rc;
+            m.vt_corners_built = true;
+            m.vt_sorted = false;
+        }
     }
     m.vv_built = true;
     if (classify) m.cp_built = true;

@@ -856,4 +856,12 @@ int run_vv_cp(VvCpBuffers &b, const VvCpConfig &cfg, bool classify,
     return rc;
 }
 
+int run_vv_cp_on_corners(VvCpBuffers &b, const VvCpConfig &cfg, bool classify,
+                         hipStream_t s) {
+    if (b.marks) b.marks->begin(s);
+    int rc = fill_i32(b.counters, 0, N_COUNTERS, s);
+    if (!rc) rc = launch_vv_cp(b, cfg, classify, s);
+    return rc;
+}
+
 }  // namespace tg
