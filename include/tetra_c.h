@@ -247,6 +247,34 @@ int tg_host_free(void *p);
 /* Free a device pointer handed out by the library to the caller. */
 int tg_device_free(void *p);
 
+/* ---- no-allocation entry point ---------------------------------------
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same results as tg_vv_critpoints +
+ * tg_relation_device(VV) + tg_classes_device on a mesh, but every pointer is
+ * a caller-owned device pointer and the call allocates nothing: its
+ * intermediates live in one static device array of 128 MiB per GPU, held
+ * until the process ends (tg_noalloc_limits reports its size and what a
+ * mesh needs; about 1.1 million cells at most).
+ *   cells [n_cells][4] int32, scalars [n_points] float32 (scalar_f64 == 0)
+ *   or float64; outputs vv_off [n_points + 1], vv_cols [vv_cap] with
+ *   vv_cap >= min(12*n_cells, n_points*(n_points-1)), classes [n_points][3];
+ *   status (optional, one int32): the device-side error bits of the run
+ *   (1: a vertex over 4096 neighbours), written at its end.
+ * Work is enqueued on `stream` (NULL: the null stream) with no host wait, so
+ * it can be captured into a graph.  Calls on one GPU share the static array
+ * and keep the last call's contents; calls that overlap on one GPU (on
+ * unordered streams) race.  Cell connectivity is not validated: ids must be
+ * in [0, n_points) and distinct within a cell.  Sizes over the limit return
+ * TG_ERR_CAPACITY before any work is enqueued. */
+int tg_vv_critpoints_noalloc(const int32_t *cells, int64_t n_cells, int64_t n_points,
+                             const void *scalars, int scalar_f64, int32_t *vv_off,
+                             int32_t *vv_cols, int64_t vv_cap, uint32_t *classes,
+                             int32_t *status, void *stream);
+/* Scratch bytes a mesh of this size needs, and the static array's size. */
+int tg_noalloc_limits(int64_t n_cells, int64_t n_points, int64_t *need_bytes,
+                      int64_t *max_bytes);
+
 #ifdef __cplusplus
 }
 #endif

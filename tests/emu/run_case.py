@@ -114,7 +114,35 @@ def case_routes(seed: int = 4, route: int = 2, bv: int = 7) -> None:
     assert np.array_equal(vt[0], ref.off) and np.array_equal(vt[1], ref.val), "VT"
 
 
-CASES = {"after_hot_path": case_after_hot_path, "all_relations": case_all_relations,
+def case_noalloc(seed: int = 6, n1: int = 7, n2: int = 4) -> None:
+    """The no-allocation entry point (host arrays stand in for device memory
+    here) against the mesh path, two calls in a row, the second smaller."""
+    C = _native.C
+    lib = _native.hip()
+    for n in (n1, n2):
+        m = t.kuhn(n, seed=seed + n)
+        with t.Mesh(m.cells, m.n_points, m.scalars) as d:
+            d.vv_critpoints()
+            off_ref, vals_ref = d.relation("VV")
+            cls_ref = d.classes()
+        T, V = m.n_cells, m.n_points
+        cells = np.ascontiguousarray(m.cells, np.int32)
+        sc = np.ascontiguousarray(m.scalars, np.float32)
+        cap = min(12 * T, V * (V - 1))
+        off = np.full(V + 1, -7, np.int32)
+        cols = np.full(cap, -7, np.int32)
+        cls = np.full((V, 3), 7, np.uint32)
+        st = np.full(1, -7, np.int32)
+        _native.check(lib.tg_vv_critpoints_noalloc(cells.ctypes.data, T, V, sc.ctypes.data, 0,
+                                                   off.ctypes.data, cols.ctypes.data, cap,
+                                                   cls.ctypes.data, st.ctypes.data, None))
+        assert st[0] == 0
+        assert np.array_equal(off, off_ref), "VV offsets"
+        assert np.array_equal(cols[: off[-1]], vals_ref[:, 0]), "VV columns"
+        assert np.array_equal(cls, cls_ref), "classes"
+
+
+CASES = {"noalloc": case_noalloc, "after_hot_path": case_after_hot_path, "all_relations": case_all_relations,
          "vv_cp": case_vv_cp, "hubs": case_hubs, "soup": case_soup, "routes": case_routes}
 
 

@@ -39,6 +39,8 @@ CASES = [
     ("hubs", 11, 40), ("soup", 2, 60, 200),
     # TE, the vertex-lane hot path, then ET/TT/VF on the same mesh
     ("after_hot_path", 8, 5, 1), ("after_hot_path", 8, 5, 2),
+    # the no-allocation entry point: two calls, the second smaller
+    ("noalloc", 6, 7, 4),
 ]
 
 

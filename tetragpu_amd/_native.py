@@ -78,6 +78,8 @@ _HIP_SIGS = {
     "tg_batch_relation_copy": (C.c_int, [P, I32, I32, P, P, C.POINTER(I64)]),
     "tg_profile_enable": (C.c_int, [P, C.c_int]),
     "tg_profile_read": (C.c_int, [P, I32, C.POINTER(I32), P, P]),
+    "tg_vv_critpoints_noalloc": (C.c_int, [P, I64, I64, P, C.c_int, P, P, I64, P, P, P]),
+    "tg_noalloc_limits": (C.c_int, [I64, I64, C.POINTER(I64), C.POINTER(I64)]),
 }
 
 _HOST_SIGS = {

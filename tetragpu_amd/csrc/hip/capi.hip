@@ -12,6 +12,8 @@
 
 namespace tg {
 
+int noalloc_init_device();  // noalloc.hip
+
 static thread_local std::string g_err;
 
 void set_error(const std::string &msg) { g_err = msg; }
@@ -168,7 +170,7 @@ int tg_init(int device) {
     if (device < 0 || device >= n) return fail(TG_ERR_ARG, "device index out of range");
     TG_HIP(hipSetDevice(device));
     TG_HIP(hipFree(nullptr));
-    return TG_OK;
+    return tg::noalloc_init_device();
 }
 
 int tg_mesh_create(tg_mesh **out, const int32_t *cells, int64_t n_cells,
