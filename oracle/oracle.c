@@ -117,6 +117,36 @@ void orc_csr_free(orc_csr *c) {
     c->rows = c->nnz = 0;
 }
 
+/* CSR from per-row counts: cnt[0..rows) becomes the offsets (rows + 1
+ * entries, cnt must hold rows + 1); values are allocated, not filled. */
+static void csr_from_counts(int64_t *cnt, int64_t rows, int32_t width, orc_csr *out) {
+    int64_t total = 0;
+    for (int64_t r = 0; r < rows; r++) {
+        const int64_t n = cnt[r];
+        cnt[r] = total;
+        total += n;
+    }
+    cnt[rows] = total;
+    out->rows = rows;
+    out->width = width;
+    out->nnz = total;
+    out->off = cnt;
+    out->val = (int64_t *)malloc((size_t)(total * width + 1) * sizeof(int64_t));
+}
+
+/* Rows r <- items i for every (r, i) pair given by key(i) over i ascending,
+ * so each row lists its items ascending (a counting sort by row). */
+static void csr_by_key(const int64_t *key, int64_t n_items, int64_t per_item_div,
+                       int64_t rows, orc_csr *out) {
+    int64_t *cnt = (int64_t *)calloc((size_t)rows + 1, sizeof(int64_t));
+    for (int64_t i = 0; i < n_items; i++) cnt[key[i]]++;
+    csr_from_counts(cnt, rows, 1, out);
+    int64_t *cur = (int64_t *)malloc(((size_t)rows + 1) * sizeof(int64_t));
+    memcpy(cur, out->off, ((size_t)rows + 1) * sizeof(int64_t));
+    for (int64_t i = 0; i < n_items; i++) out->val[cur[key[i]]++] = i / per_item_div;
+    free(cur);
+}
+
 static int cells_valid(const int64_t *cells, int64_t n_cells, int64_t n_points) {
     for (int64_t i = 0; i < 4 * n_cells; i++)
         if (cells[i] < 0 || cells[i] >= n_points) return 0;
@@ -196,10 +226,8 @@ void orc_make_EV(const orc_csr *VE, int64_t n_edges, int64_t *EV) {
 
 void orc_make_ET(const int64_t *TE, int64_t n_cells, int64_t n_edges,
                  orc_csr *ET) {
-    vec *rows = (vec *)calloc((size_t)n_edges + 1, sizeof(vec));
-    for (int64_t c = 0; c < n_cells; c++)
-        for (int k = 0; k < 6; k++) vpush(&rows[TE[6 * c + k]], c);
-    flatten(rows, n_edges, 1, ET);
+    /* cell c is appended to the rows of its six edges, cells ascending */
+    csr_by_key(TE, 6 * n_cells, 6, n_edges, ET);
 }
 
 void orc_make_FV(const orc_csr *VF, int64_t n_faces, int64_t *FV) {
@@ -235,10 +263,8 @@ void orc_make_VV(const int64_t *cells, int64_t n_cells, int64_t n_points,
 
 void orc_make_VT(const int64_t *cells, int64_t n_cells, int64_t n_points,
                  orc_csr *VT) {
-    vec *rows = (vec *)calloc((size_t)n_points + 1, sizeof(vec));
-    for (int64_t c = 0; c < n_cells; c++)
-        for (int i = 0; i < 4; i++) vpush(&rows[cells[4 * c + i]], c);
-    flatten(rows, n_points, 1, VT);
+    /* cell c is appended to the rows of its four vertices, cells ascending */
+    csr_by_key(cells, 4 * n_cells, 4, n_points, VT);
 }
 
 /* ---- relations defined here (no reference implementation) ------------ */
@@ -251,17 +277,19 @@ static int64_t ve_lookup(const orc_csr *VE, int64_t a, int64_t b) {
 
 int orc_make_FE(const int64_t *FV, int64_t n_faces, const orc_csr *VE,
                 int64_t *FE) {
+    int missing = 0;
+#pragma omp parallel for schedule(static) reduction(| : missing)
     for (int64_t f = 0; f < n_faces; f++) {
         int64_t mid = FV[3 * f], hi = FV[3 * f + 1], lo = FV[3 * f + 2];
         int64_t p[3][2] = {{lo, mid}, {lo, hi}, {mid, hi}};
         for (int e = 0; e < 3; e++) {
             int64_t id = ve_lookup(VE, p[e][0], p[e][1]);
-            if (id < 0) return -1;
+            missing |= id < 0;
             FE[6 * f + 2 * e] = p[e][0] > p[e][1] ? p[e][0] : p[e][1];
             FE[6 * f + 2 * e + 1] = id;
         }
     }
-    return 0;
+    return missing ? -1 : 0;
 }
 
 int orc_make_FT(const int64_t *TF, int64_t n_cells, int64_t n_faces,
@@ -280,23 +308,70 @@ int orc_make_FT(const int64_t *TF, int64_t n_cells, int64_t n_faces,
 
 void orc_make_EF(const int64_t *FE, int64_t n_faces, int64_t n_edges,
                  orc_csr *EF) {
-    vec *rows = (vec *)calloc((size_t)n_edges + 1, sizeof(vec));
+    /* face f is appended once to the row of each distinct edge it holds,
+     * faces ascending */
+    int64_t *cnt = (int64_t *)calloc((size_t)n_edges + 1, sizeof(int64_t));
     for (int64_t f = 0; f < n_faces; f++)
         for (int e = 0; e < 3; e++) {
-            vec *L = &rows[FE[6 * f + 2 * e + 1]];
-            if (L->n == 0 || L->d[L->n - 1] != f) vpush(L, f);
+            const int64_t id = FE[6 * f + 2 * e + 1];
+            int dup = 0;
+            for (int x = 0; x < e; x++) dup |= FE[6 * f + 2 * x + 1] == id;
+            if (!dup) cnt[id]++;
         }
-    flatten(rows, n_edges, 1, EF);
+    csr_from_counts(cnt, n_edges, 1, EF);
+    int64_t *cur = (int64_t *)malloc(((size_t)n_edges + 1) * sizeof(int64_t));
+    memcpy(cur, EF->off, ((size_t)n_edges + 1) * sizeof(int64_t));
+    for (int64_t f = 0; f < n_faces; f++)
+        for (int e = 0; e < 3; e++) {
+            const int64_t id = FE[6 * f + 2 * e + 1];
+            int dup = 0;
+            for (int x = 0; x < e; x++) dup |= FE[6 * f + 2 * x + 1] == id;
+            if (!dup) EF->val[cur[id]++] = f;
+        }
+    free(cur);
 }
 
-static int cmp_i64(const void *a, const void *b) {
-    int64_t x = *(const int64_t *)a, y = *(const int64_t *)b;
-    return (x > y) - (x < y);
+/* Ascending sort of a short array: quicksort (middle pivot) down to runs of
+ * 16, then insertion sort. */
+static void sort_i64(int64_t *a, int64_t n) {
+    while (n > 16) {
+        const int64_t p = a[n / 2];
+        int64_t i = 0, j = n - 1;
+        while (i <= j) {
+            while (a[i] < p) i++;
+            while (a[j] > p) j--;
+            if (i <= j) {
+                const int64_t t = a[i];
+                a[i] = a[j];
+                a[j] = t;
+                i++;
+                j--;
+            }
+        }
+        /* recurse into the smaller side, loop on the larger */
+        if (j + 1 < n - i) {
+            sort_i64(a, j + 1);
+            a += i;
+            n -= i;
+        } else {
+            sort_i64(a + i, n - i);
+            n = j + 1;
+        }
+    }
+    for (int64_t i = 1; i < n; i++) {
+        const int64_t x = a[i];
+        int64_t j = i - 1;
+        while (j >= 0 && a[j] > x) {
+            a[j + 1] = a[j];
+            j--;
+        }
+        a[j + 1] = x;
+    }
 }
 
 /* Sort, drop duplicates and drop `self`; returns the new length. */
 static int64_t sort_unique_drop(int64_t *buf, int64_t n, int64_t self) {
-    qsort(buf, (size_t)n, sizeof(int64_t), cmp_i64);
+    sort_i64(buf, n);
     int64_t m = 0;
     for (int64_t i = 0; i < n; i++) {
         if (buf[i] == self) continue;
@@ -306,66 +381,104 @@ static int64_t sort_unique_drop(int64_t *buf, int64_t n, int64_t self) {
     return m;
 }
 
+/* Row-parallel CSR build: row(r, buf) writes row r (sorted, unique) into a
+ * thread's buffer and returns its length; rows are computed twice, once to
+ * count and once to fill, so every row is the same computation. */
+typedef int64_t (*row_fn)(const void *ctx, int64_t r, vec *buf);
+
+static void csr_by_rows(const void *ctx, row_fn row, int64_t rows, orc_csr *out) {
+    int64_t *cnt = (int64_t *)calloc((size_t)rows + 1, sizeof(int64_t));
+#pragma omp parallel
+    {
+        vec buf = {0, 0, 0};
+#pragma omp for schedule(dynamic, 4096)
+        for (int64_t r = 0; r < rows; r++) cnt[r] = row(ctx, r, &buf);
+        free(buf.d);
+    }
+    csr_from_counts(cnt, rows, 1, out);
+#pragma omp parallel
+    {
+        vec buf = {0, 0, 0};
+#pragma omp for schedule(dynamic, 4096)
+        for (int64_t r = 0; r < rows; r++) {
+            const int64_t m = row(ctx, r, &buf);
+            if (m) memcpy(out->val + out->off[r], buf.d, (size_t)m * sizeof(int64_t));
+        }
+        free(buf.d);
+    }
+}
+
+typedef struct {
+    const int64_t *cells;
+    orc_csr VT;
+} tt_ctx;
+
+/* TT[t]: the cells of t's four VT rows, sorted, unique, without t */
+static int64_t tt_row(const void *p, int64_t t, vec *buf) {
+    const tt_ctx *c = (const tt_ctx *)p;
+    buf->n = 0;
+    for (int i = 0; i < 4; i++) {
+        const int64_t v = c->cells[4 * t + i];
+        for (int64_t k = c->VT.off[v]; k < c->VT.off[v + 1]; k++) vpush(buf, c->VT.val[k]);
+    }
+    return sort_unique_drop(buf->d, buf->n, t);
+}
+
 void orc_make_TT(const int64_t *cells, int64_t n_cells, int64_t n_points,
                  orc_csr *TT) {
-    orc_csr VT;
-    orc_make_VT(cells, n_cells, n_points, &VT);
-    vec *rows = (vec *)calloc((size_t)n_cells + 1, sizeof(vec));
-    vec tmp = {0, 0, 0};
-    for (int64_t t = 0; t < n_cells; t++) {
-        tmp.n = 0;
-        for (int i = 0; i < 4; i++) {
-            int64_t v = cells[4 * t + i];
-            for (int64_t k = VT.off[v]; k < VT.off[v + 1]; k++) vpush(&tmp, VT.val[k]);
-        }
-        int64_t m = sort_unique_drop(tmp.d, tmp.n, t);
-        for (int64_t k = 0; k < m; k++) vpush(&rows[t], tmp.d[k]);
+    tt_ctx c;
+    c.cells = cells;
+    orc_make_VT(cells, n_cells, n_points, &c.VT);
+    csr_by_rows(&c, tt_row, n_cells, TT);
+    orc_csr_free(&c.VT);
+}
+
+typedef struct {
+    const int64_t *TF, *FT;
+} ff_ctx;
+
+/* FF[f]: the faces of f's (at most two) cells, sorted, unique, without f */
+static int64_t ff_row(const void *p, int64_t f, vec *buf) {
+    const ff_ctx *c = (const ff_ctx *)p;
+    buf->n = 0;
+    for (int j = 0; j < 2; j++) {
+        const int64_t t = c->FT[2 * f + j];
+        if (t < 0) continue;
+        for (int s = 0; s < 4; s++) vpush(buf, c->TF[4 * t + s]);
     }
-    free(tmp.d);
-    orc_csr_free(&VT);
-    flatten(rows, n_cells, 1, TT);
+    return sort_unique_drop(buf->d, buf->n, f);
 }
 
 void orc_make_FF(const int64_t *TF, int64_t n_cells, const int64_t *FT,
                  int64_t n_faces, orc_csr *FF) {
     (void)n_cells;
-    vec *rows = (vec *)calloc((size_t)n_faces + 1, sizeof(vec));
-    int64_t buf[8];
-    for (int64_t f = 0; f < n_faces; f++) {
-        int64_t n = 0;
-        for (int j = 0; j < 2; j++) {
-            int64_t t = FT[2 * f + j];
-            if (t < 0) continue;
-            for (int s = 0; s < 4; s++) buf[n++] = TF[4 * t + s];
-        }
-        int64_t m = sort_unique_drop(buf, n, f);
-        for (int64_t k = 0; k < m; k++) vpush(&rows[f], buf[k]);
+    ff_ctx c = {TF, FT};
+    csr_by_rows(&c, ff_row, n_faces, FF);
+}
+
+typedef struct {
+    const int64_t *EV;
+    orc_csr inc;  /* vertex -> incident edges, ascending */
+} ee_ctx;
+
+/* EE[e]: the edges at either endpoint of e, sorted, unique, without e */
+static int64_t ee_row(const void *p, int64_t e, vec *buf) {
+    const ee_ctx *c = (const ee_ctx *)p;
+    buf->n = 0;
+    for (int j = 0; j < 2; j++) {
+        const int64_t v = c->EV[2 * e + j];
+        for (int64_t k = c->inc.off[v]; k < c->inc.off[v + 1]; k++) vpush(buf, c->inc.val[k]);
     }
-    flatten(rows, n_faces, 1, FF);
+    return sort_unique_drop(buf->d, buf->n, e);
 }
 
 void orc_make_EE(const int64_t *EV, int64_t n_edges, int64_t n_points,
                  orc_csr *EE) {
-    vec *inc = (vec *)calloc((size_t)n_points + 1, sizeof(vec));
-    for (int64_t e = 0; e < n_edges; e++) {
-        vpush(&inc[EV[2 * e]], e);
-        vpush(&inc[EV[2 * e + 1]], e);
-    }
-    vec *rows = (vec *)calloc((size_t)n_edges + 1, sizeof(vec));
-    vec tmp = {0, 0, 0};
-    for (int64_t e = 0; e < n_edges; e++) {
-        tmp.n = 0;
-        for (int j = 0; j < 2; j++) {
-            vec *L = &inc[EV[2 * e + j]];
-            for (int64_t k = 0; k < L->n; k++) vpush(&tmp, L->d[k]);
-        }
-        int64_t m = sort_unique_drop(tmp.d, tmp.n, e);
-        for (int64_t k = 0; k < m; k++) vpush(&rows[e], tmp.d[k]);
-    }
-    free(tmp.d);
-    for (int64_t v = 0; v < n_points; v++) free(inc[v].d);
-    free(inc);
-    flatten(rows, n_edges, 1, EE);
+    ee_ctx c;
+    c.EV = EV;
+    csr_by_key(EV, 2 * n_edges, 2, n_points, &c.inc);
+    csr_by_rows(&c, ee_row, n_edges, EE);
+    orc_csr_free(&c.inc);
 }
 
 /* ---- critical points (critPoints.cu:49-338) --------------------------- */
