@@ -8,6 +8,9 @@
 //   dropin_probe load <input> [<arrayname> [<partitioningname>]]
 //   dropin_probe part <cells.i64> <n_points> <scalars.f64> <parts.i32> <cpu|gpu>
 //                [<partcells.i32>]
+//   dropin_probe gpuids <cells.i64> <n_points> <edge_perm.i64> <face_perm.i64> [bad]
+//                (GPU builders given host VE/VF renumbered by the permutations;
+//                 "bad" moves one VE entry to an edge the mesh does not have)
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
@@ -17,8 +20,13 @@
 
 #include "argparse.h"
 #include "cpu_extraction.h"
+#include "cuda_extraction.h"
 #include "critpoints.h"
 #include "vtk_load.h"
+
+// HIP runtime entry used to read back the _return builders' device arrays
+// (a reference caller does the same with cudaMemcpy); linked from libamdhip64.
+extern "C" int hipMemcpy(void *dst, const void *src, size_t bytes, int kind);
 
 namespace {
 
@@ -202,6 +210,67 @@ int part_cmd(int argc, char **argv) {
     return 0;
 }
 
+template <size_t W>
+void fixed_ptr(const char *name, const vtkIdType *d, size_t rows) {
+    std::vector<std::array<vtkIdType, W>> h(rows);
+    if (rows && hipMemcpy(h.data(), d, rows * W * sizeof(vtkIdType), 2 /* device to host */)) {
+        std::fprintf(stderr, "hipMemcpy failed\n");
+        std::exit(3);
+    }
+    tg_device_free((void *)d);
+    fixed(name, h);
+}
+
+int gpuids_cmd(int argc, char **argv) {
+    auto cells = read_file<int64_t>(argv[2]);
+    const vtkIdType V = std::atoll(argv[3]);
+    auto eperm = read_file<int64_t>(argv[4]);
+    auto fperm = read_file<int64_t>(argv[5]);
+    const bool bad = argc > 6 && std::string(argv[6]) == "bad";
+    TV_Data tv(V, (vtkIdType)cells.size() / 4);
+    std::memcpy(tv.cells, cells.data(), cells.size() * sizeof(int64_t));
+    TE_Data TE0;
+    VE_Data VE;
+    TF_Data TF0;
+    VF_Data VF;
+    const vtkIdType E = make_TE_and_VE(tv, TE0, VE);
+    const vtkIdType F = make_TF_and_VF(tv, TF0, VF);
+    for (auto &row : VE)
+        for (auto &e : row) e.id = eperm[e.id];
+    for (auto &row : VF)
+        for (auto &f : row) f.id = fperm[f.id];
+    if (bad)
+        for (auto &row : VE)
+            if (!row.empty()) {
+                row[0].highVert = V + 7;  // an edge no cell holds
+                break;
+            }
+    const vtkIdType T = tv.nCells;
+    make_TV_for_GPU(tv);
+    make_VE_for_GPU(VE, V, E);  // the reference's call order (src/cuda_extraction.cu:271)
+    make_VF_for_GPU(VF, V, F);
+    std::cout << "{\n";
+    fixed("EV", *make_EV_GPU(VE, V, E, false));
+    fixed("TE", *make_TE_GPU(tv, VE, V, E, T, false));
+    fixed("TF", *make_TF_GPU(tv, VF, V, F, T, false));
+    auto FV = make_FV_GPU(VF, V, F, false);
+    std::vector<std::array<vtkIdType, 3>> fv(FV->size());
+    for (size_t f = 0; f < FV->size(); f++) fv[f] = {(*FV)[f].middleVert, (*FV)[f].highVert, (*FV)[f].id};
+    fixed("FV", fv);
+    auto FE = make_FE_GPU(VF, VE, V, E, F, false);
+    std::vector<std::array<vtkIdType, 6>> fe(FE->size());
+    for (size_t f = 0; f < FE->size(); f++)
+        for (int k = 0; k < 3; k++) fe[f][2 * k] = (*FE)[f][k].highVert, fe[f][2 * k + 1] = (*FE)[f][k].id;
+    fixed("FE", fe);
+    fixed_ptr<2>("EV_dev", make_EV_GPU_return(VE, V, E, false), (size_t)E);
+    fixed_ptr<6>("TE_dev", make_TE_GPU_return(tv, VE, V, E, T, false), (size_t)T);
+    fixed_ptr<4>("TF_dev", make_TF_GPU_return(tv, VF, V, F, T, false), (size_t)T);
+    fixed_ptr<3>("FV_dev", make_FV_GPU_return(VF, V, F, false), (size_t)F);
+    fixed_ptr<6>("FE_dev", make_FE_GPU_return(VF, VE, V, E, F, true), (size_t)F);
+    std::cout << "\"E\": " << E << ", \"F\": " << F << "\n}\n";
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -212,5 +281,6 @@ int main(int argc, char **argv) {
     if (cmd == "export") return export_cmd(argv);
     if (cmd == "load") return load_cmd(argc, argv);
     if (cmd == "part") return part_cmd(argc, argv);
+    if (cmd == "gpuids") return gpuids_cmd(argc, argv);
     return 2;
 }

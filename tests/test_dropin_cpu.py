@@ -26,7 +26,8 @@ def probe(tmp_path_factory):
     exe = tmp_path_factory.mktemp("probe") / "dropin_probe"
     cmd = ["g++", "-O1", "-std=c++17", f"-I{ROOT / 'include' / 'tetra'}", f"-I{ROOT / 'include'}",
            str(ROOT / "tests" / "cpp" / "dropin_probe.cpp"), "-o", str(exe), f"-L{LIB}",
-           "-ltetra_host", "-ltetra_hip", f"-Wl,-rpath,{LIB}"]
+           "-ltetra_host", "-ltetra_hip", "-L/opt/rocm/lib", "-lamdhip64",
+           f"-Wl,-rpath,{LIB}", "-Wl,-rpath,/opt/rocm/lib"]
     subprocess.run(cmd, check=True, capture_output=True, text=True)
     return exe
 

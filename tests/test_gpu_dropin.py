@@ -106,3 +106,60 @@ def test_tetra_main_partitions_from_vtu(gpu, tmp_path):
         line = f"Partition {p} number of saddles:"
         got = [int(l.split(":")[1]) for l in text.splitlines() if l.startswith(line)]
         assert got == [int(((cls[:, 2] == 4) & (parts == p)).sum())]
+
+
+def perm_files(tmp_path, E, F, seed):
+    rng = np.random.default_rng(seed)
+    ep, fp = rng.permutation(E).astype(np.int64), rng.permutation(F).astype(np.int64)
+    (tmp_path / "ep.i64").write_bytes(ep.tobytes())
+    (tmp_path / "fp.i64").write_bytes(fp.tobytes())
+    return ep, fp
+
+
+def test_builders_honour_caller_numbering(gpu, probe, tmp_path):
+    """make_{EV,TE,TF,FV,FE}_GPU(_return) given VE/VF whose ids are renumbered
+    return their results in the caller's numbering, as the reference's
+    builders do by computing from the uploaded tables
+    (src/cuda_extraction.cu:85-252, 254-343, 491-844)."""
+    import json
+    from test_dropin_cpu import run
+
+    m = gpu.kuhn(4, seed=9, order="shuffled")
+    cells = np.ascontiguousarray(m.cells, np.int64)
+    (tmp_path / "cells.i64").write_bytes(cells.tobytes())
+    E, TE, VE = O.edges(m.cells, m.n_points)
+    F, TF, VF = O.faces(m.cells, m.n_points)
+    EV, FV = O.ev(VE, E), O.fv(VF, F)
+    FE = O.fe(FV, VE).reshape(F, 6)
+    for seed, identity in ((1, True), (2, False)):
+        ep, fp = perm_files(tmp_path, E, F, seed)
+        if identity:
+            ep, fp = np.arange(E), np.arange(F)
+            (tmp_path / "ep.i64").write_bytes(ep.astype(np.int64).tobytes())
+            (tmp_path / "fp.i64").write_bytes(fp.astype(np.int64).tobytes())
+        r = run(probe, "gpuids", tmp_path / "cells.i64", m.n_points, tmp_path / "ep.i64",
+                tmp_path / "fp.i64")
+        got = json.loads(r.stdout)
+        ev = np.empty_like(EV); ev[ep] = EV
+        fv = np.empty_like(FV); fv[fp] = FV
+        fe = np.empty_like(FE); fe[fp] = FE
+        fe[:, 1::2] = ep[fe[:, 1::2]]
+        for k in ("", "_dev"):
+            assert np.array_equal(got["EV" + k], ev)
+            assert np.array_equal(got["TE" + k], ep[TE])
+            assert np.array_equal(got["TF" + k], fp[TF])
+            assert np.array_equal(got["FV" + k], fv)
+            assert np.array_equal(got["FE" + k], fe)
+
+
+def test_builders_reject_foreign_table(gpu, probe, tmp_path):
+    from test_dropin_cpu import run
+
+    m = gpu.kuhn(3, seed=4)
+    (tmp_path / "cells.i64").write_bytes(np.ascontiguousarray(m.cells, np.int64).tobytes())
+    E, _, _ = O.edges(m.cells, m.n_points)
+    F, _, _ = O.faces(m.cells, m.n_points)
+    perm_files(tmp_path, E, F, 3)
+    r = run(probe, "gpuids", tmp_path / "cells.i64", m.n_points, tmp_path / "ep.i64",
+            tmp_path / "fp.i64", "bad", check=False)
+    assert r.returncode != 0 and "does not describe this mesh" in r.stderr

@@ -14,11 +14,25 @@
 
 namespace {
 
+// The caller's edge or face numbering, as a map from the device's ids (the
+// first-occurrence order of make_TE_and_VE / make_TF_and_VF) to the ids in
+// the caller's VE or VF table.  The reference uploads the caller's table and
+// computes every id relation from it (src/cuda_extraction.cu:85-252, used by
+// EV/TF/TE/FV/FE at :271, :409, :632, :770), so its results carry the
+// caller's ids; here the device's results are renumbered through this map.
+struct IdMap {
+    const void *table = nullptr;  // the VE_Data / VF_Data it was made from
+    size_t rows = 0;
+    std::vector<int64_t> to_caller;
+    bool identity = true;
+};
+
 struct Session {
     tg_mesh *mesh = nullptr;
     const TV_Data *tv = nullptr;  // the TV the mesh was made from
     vtkIdType n_points = 0, n_cells = 0;
     bool initialised = false;
+    IdMap edges, faces;
 };
 
 thread_local Session g_session;
@@ -36,6 +50,8 @@ void release_session() {
     if (g_session.mesh) tg_mesh_destroy(g_session.mesh);
     g_session.mesh = nullptr;
     g_session.tv = nullptr;
+    g_session.edges = IdMap();
+    g_session.faces = IdMap();
 }
 
 // (Re)create the thread's mesh from TV: cells narrowed to int32 with a range
@@ -149,6 +165,111 @@ int64_t n_faces(tg_mesh *m) {
     return f;
 }
 
+[[noreturn]] void table_mismatch(const char *what, vtkIdType v, const char *detail) {
+    std::fprintf(stderr, "tetra: the caller's %s does not describe this mesh (vertex %lld: %s)\n",
+                 what, (long long)v, detail);
+    std::exit(EXIT_FAILURE);
+}
+
+// Map device edge ids to the ids of the caller's VE: row v of VE must hold
+// exactly the device's edges (v, hi) with hi > v, in any order, and the ids
+// must be a permutation of [0, E).
+const IdMap &edge_map(tg_mesh *m, const VE_Data &ve) {
+    IdMap &M = g_session.edges;
+    if (M.table == &ve && M.rows == ve.size()) return M;
+    build(m, TG_REL_VE);
+    const HostRel d = fetch(m, TG_REL_VE);  // rows ascending by id: {hi, id}
+    if ((int64_t)ve.size() != d.rows) table_mismatch("VE", -1, "row count differs");
+    const int64_t E = d.nnz;
+    std::vector<int64_t> map((size_t)E, -1);
+    std::vector<char> seen((size_t)E, 0);
+    bool identity = true;
+    for (int64_t v = 0; v < d.rows; v++) {
+        const auto &row = ve[(size_t)v];
+        if ((int64_t)row.size() != d.off[v + 1] - d.off[v]) table_mismatch("VE", v, "row length differs");
+        for (int64_t i = d.off[v]; i < d.off[v + 1]; i++) {
+            const int64_t hi = d.val[2 * i], id = d.val[2 * i + 1];
+            int64_t cid = -1;
+            for (const EdgeData &e : row)
+                if (e.highVert == hi) {
+                    cid = e.id;
+                    break;
+                }
+            if (cid < 0) table_mismatch("VE", v, "an edge of the mesh is missing");
+            if (cid >= E || seen[(size_t)cid]) table_mismatch("VE", v, "ids are not a permutation of [0, E)");
+            seen[(size_t)cid] = 1;
+            map[(size_t)id] = cid;
+            identity = identity && cid == id;
+        }
+    }
+    M.table = &ve;
+    M.rows = ve.size();
+    M.to_caller = std::move(map);
+    M.identity = identity;
+    return M;
+}
+
+// Map device face ids to the caller's VF ids: row v of VF must hold exactly
+// the device's faces whose positional low vertex is v, matched by (mid, hi).
+const IdMap &face_map(tg_mesh *m, const VF_Data &vf) {
+    IdMap &M = g_session.faces;
+    if (M.table == &vf && M.rows == vf.size()) return M;
+    build(m, TG_REL_VF);
+    const HostRel d = fetch(m, TG_REL_VF);  // {mid, hi, id}
+    if ((int64_t)vf.size() != d.rows) table_mismatch("VF", -1, "row count differs");
+    const int64_t F = d.nnz;
+    std::vector<int64_t> map((size_t)F, -1);
+    std::vector<char> seen((size_t)F, 0);
+    bool identity = true;
+    for (int64_t v = 0; v < d.rows; v++) {
+        const auto &row = vf[(size_t)v];
+        if ((int64_t)row.size() != d.off[v + 1] - d.off[v]) table_mismatch("VF", v, "row length differs");
+        for (int64_t i = d.off[v]; i < d.off[v + 1]; i++) {
+            const int64_t mid = d.val[3 * i], hi = d.val[3 * i + 1], id = d.val[3 * i + 2];
+            int64_t cid = -1;
+            for (const FaceData &f : row)
+                if (f.middleVert == mid && f.highVert == hi) {
+                    cid = f.id;
+                    break;
+                }
+            if (cid < 0) table_mismatch("VF", v, "a face of the mesh is missing");
+            if (cid >= F || seen[(size_t)cid]) table_mismatch("VF", v, "ids are not a permutation of [0, F)");
+            seen[(size_t)cid] = 1;
+            map[(size_t)id] = cid;
+            identity = identity && cid == id;
+        }
+    }
+    M.table = &vf;
+    M.rows = vf.size();
+    M.to_caller = std::move(map);
+    M.identity = identity;
+    return M;
+}
+
+// Device relation in the caller's numbering: rows indexed by a renumbered id
+// move to their new index (row_map), and value columns holding ids are
+// renumbered (val_map; for FE only the id column of each {hi, id} pair).
+std::vector<int64_t> renumbered(const HostRel &h, const IdMap *row_map, const IdMap *val_map,
+                                int val_stride = 1, int val_col = 0) {
+    const size_t W = (size_t)h.width;
+    std::vector<int64_t> out(h.val.size());
+    for (int64_t r = 0; r < h.rows; r++) {
+        const size_t dst = row_map ? (size_t)row_map->to_caller[(size_t)r] : (size_t)r;
+        for (size_t k = 0; k < W; k++) {
+            int64_t x = h.val[(size_t)r * W + k];
+            if (val_map && (int)(k % val_stride) == val_col) x = val_map->to_caller[(size_t)x];
+            out[dst * W + k] = x;
+        }
+    }
+    return out;
+}
+
+vtkIdType *upload_values(const std::vector<int64_t> &v) {
+    void *d = nullptr;
+    ok(tg_device_upload(v.data(), v.size() * sizeof(int64_t), &d), "tg_device_upload");
+    return (vtkIdType *)d;
+}
+
 void finish(bool free_transients) {
     if (free_transients) release_session();
 }
@@ -176,94 +297,132 @@ tg_mesh *tetra_current_mesh() { return g_session.mesh; }
 
 void make_TV_for_GPU(const TV_Data &tv_relationship) { upload(tv_relationship); }
 
-void make_VE_for_GPU(const VE_Data &ve, const vtkIdType n_edges_, const vtkIdType n_verts) {
+// The reference declares (table, n_edges, n_verts) (include/cuda_extraction.h:
+// 15-17) but defines and calls it as (table, n_verts, n_edges)
+// (src/cuda_extraction.cu:85-88, :271); the count equal to the table's row
+// count is taken as the vertex count, the other as the edge count.
+void make_VE_for_GPU(const VE_Data &ve, const vtkIdType a, const vtkIdType b) {
     tg_mesh *m = current("make_VE_for_GPU");
-    expect_count("vertices of VE", (vtkIdType)ve.size(), n_verts);
-    expect_count("edges", n_edges(m), n_edges_);
+    const vtkIdType n_verts = (vtkIdType)ve.size();
+    if (a != n_verts && b != n_verts) expect_count("vertices of VE", n_verts, b);
+    expect_count("edges", n_edges(m), a == n_verts ? b : a);
+    edge_map(m, ve);
 }
 
 void make_VF_for_GPU(const VF_Data &vf, const vtkIdType n_verts, const vtkIdType n_faces_) {
     tg_mesh *m = current("make_VF_for_GPU");
     expect_count("vertices of VF", (vtkIdType)vf.size(), n_verts);
     expect_count("faces", n_faces(m), n_faces_);
+    face_map(m, vf);
 }
 
-vtkIdType *make_EV_GPU_return(const VE_Data &, const vtkIdType, const vtkIdType n_edges_,
+vtkIdType *make_EV_GPU_return(const VE_Data &VE, const vtkIdType, const vtkIdType n_edges_,
                               const bool free_transients) {
     tg_mesh *m = current("make_EV_GPU_return");
     expect_count("edges", n_edges(m), n_edges_);
-    vtkIdType *out = export_values(m, TG_REL_EV);
+    const IdMap &em = edge_map(m, VE);
+    vtkIdType *out = em.identity ? export_values(m, TG_REL_EV)
+                                 : upload_values(renumbered(fetch(m, TG_REL_EV), &em, nullptr));
     finish(free_transients);
     return out;
 }
 
-std::unique_ptr<EV_Data> make_EV_GPU(const VE_Data &, const vtkIdType, const vtkIdType n_edges_,
+std::unique_ptr<EV_Data> make_EV_GPU(const VE_Data &VE, const vtkIdType, const vtkIdType n_edges_,
                                      const bool free_transients) {
     tg_mesh *m = current("make_EV_GPU");
     expect_count("edges", n_edges(m), n_edges_);
-    auto out = std::make_unique<EV_Data>(to_fixed<2>(fetch(m, TG_REL_EV)));
+    const IdMap &em = edge_map(m, VE);
+    HostRel h = fetch(m, TG_REL_EV);
+    if (!em.identity) {
+        const auto v = renumbered(h, &em, nullptr);
+        h.val.assign(v.begin(), v.end());
+    }
+    auto out = std::make_unique<EV_Data>(to_fixed<2>(h));
     finish(free_transients);
     return out;
 }
 
-vtkIdType *make_TF_GPU_return(const TV_Data &TV, const VF_Data &, const vtkIdType,
+vtkIdType *make_TF_GPU_return(const TV_Data &TV, const VF_Data &VF, const vtkIdType,
                               const vtkIdType n_faces_, const vtkIdType,
                               const bool free_transients) {
     tg_mesh *m = mesh_for(TV);
     expect_count("faces", n_faces(m), n_faces_);
+    const IdMap &fm = face_map(m, VF);
     build(m, TG_REL_TF);
-    vtkIdType *out = export_values(m, TG_REL_TF);
+    vtkIdType *out = fm.identity ? export_values(m, TG_REL_TF)
+                                 : upload_values(renumbered(fetch(m, TG_REL_TF), nullptr, &fm));
     finish(free_transients);
     return out;
 }
 
-std::unique_ptr<TF_Data> make_TF_GPU(const TV_Data &TV, const VF_Data &, const vtkIdType,
+std::unique_ptr<TF_Data> make_TF_GPU(const TV_Data &TV, const VF_Data &VF, const vtkIdType,
                                      const vtkIdType n_faces_, const vtkIdType,
                                      const bool free_transients) {
     tg_mesh *m = mesh_for(TV);
     expect_count("faces", n_faces(m), n_faces_);
+    const IdMap &fm = face_map(m, VF);
     build(m, TG_REL_TF);
-    auto out = std::make_unique<TF_Data>(to_fixed<4>(fetch(m, TG_REL_TF)));
+    HostRel h = fetch(m, TG_REL_TF);
+    if (!fm.identity) {
+        const auto v = renumbered(h, nullptr, &fm);
+        h.val.assign(v.begin(), v.end());
+    }
+    auto out = std::make_unique<TF_Data>(to_fixed<4>(h));
     finish(free_transients);
     return out;
 }
 
-vtkIdType *make_TE_GPU_return(const TV_Data &TV, const VE_Data &, const vtkIdType,
+vtkIdType *make_TE_GPU_return(const TV_Data &TV, const VE_Data &VE, const vtkIdType,
                               const vtkIdType n_edges_, const vtkIdType,
                               const bool free_transients) {
     tg_mesh *m = mesh_for(TV);
     expect_count("edges", n_edges(m), n_edges_);
+    const IdMap &em = edge_map(m, VE);
     build(m, TG_REL_TE);
-    vtkIdType *out = export_values(m, TG_REL_TE);
+    vtkIdType *out = em.identity ? export_values(m, TG_REL_TE)
+                                 : upload_values(renumbered(fetch(m, TG_REL_TE), nullptr, &em));
     finish(free_transients);
     return out;
 }
 
-std::unique_ptr<TE_Data> make_TE_GPU(const TV_Data &TV, const VE_Data &, const vtkIdType,
+std::unique_ptr<TE_Data> make_TE_GPU(const TV_Data &TV, const VE_Data &VE, const vtkIdType,
                                      const vtkIdType n_edges_, const vtkIdType,
                                      const bool free_transients) {
     tg_mesh *m = mesh_for(TV);
     expect_count("edges", n_edges(m), n_edges_);
+    const IdMap &em = edge_map(m, VE);
     build(m, TG_REL_TE);
-    auto out = std::make_unique<TE_Data>(to_fixed<6>(fetch(m, TG_REL_TE)));
+    HostRel h = fetch(m, TG_REL_TE);
+    if (!em.identity) {
+        const auto v = renumbered(h, nullptr, &em);
+        h.val.assign(v.begin(), v.end());
+    }
+    auto out = std::make_unique<TE_Data>(to_fixed<6>(h));
     finish(free_transients);
     return out;
 }
 
-vtkIdType *make_FV_GPU_return(const VF_Data &, const vtkIdType, const vtkIdType n_faces_,
+vtkIdType *make_FV_GPU_return(const VF_Data &VF, const vtkIdType, const vtkIdType n_faces_,
                               const bool free_transients) {
     tg_mesh *m = current("make_FV_GPU_return");
     expect_count("faces", n_faces(m), n_faces_);
-    vtkIdType *out = export_values(m, TG_REL_FV);
+    const IdMap &fm = face_map(m, VF);
+    vtkIdType *out = fm.identity ? export_values(m, TG_REL_FV)
+                                 : upload_values(renumbered(fetch(m, TG_REL_FV), &fm, nullptr));
     finish(free_transients);
     return out;
 }
 
-std::unique_ptr<FV_Data> make_FV_GPU(const VF_Data &, const vtkIdType, const vtkIdType n_faces_,
+std::unique_ptr<FV_Data> make_FV_GPU(const VF_Data &VF, const vtkIdType, const vtkIdType n_faces_,
                                      const bool free_transients) {
     tg_mesh *m = current("make_FV_GPU");
     expect_count("faces", n_faces(m), n_faces_);
-    const HostRel h = fetch(m, TG_REL_FV);
+    const IdMap &fm = face_map(m, VF);
+    HostRel h = fetch(m, TG_REL_FV);
+    if (!fm.identity) {
+        const auto v = renumbered(h, &fm, nullptr);
+        h.val.assign(v.begin(), v.end());
+    }
     auto out = std::make_unique<FV_Data>((size_t)h.rows);
     for (int64_t f = 0; f < h.rows; f++)
         (*out)[f] = FaceData(h.val[3 * f], h.val[3 * f + 1], h.val[3 * f + 2]);
@@ -271,26 +430,36 @@ std::unique_ptr<FV_Data> make_FV_GPU(const VF_Data &, const vtkIdType, const vtk
     return out;
 }
 
-vtkIdType *make_FE_GPU_return(const VF_Data &, const VE_Data &, const vtkIdType,
+vtkIdType *make_FE_GPU_return(const VF_Data &VF, const VE_Data &VE, const vtkIdType,
                               const vtkIdType n_edges_, const vtkIdType n_faces_,
                               const bool free_transients) {
     tg_mesh *m = current("make_FE_GPU_return");
     expect_count("edges", n_edges(m), n_edges_);
     expect_count("faces", n_faces(m), n_faces_);
+    const IdMap &em = edge_map(m, VE);
+    const IdMap &fm = face_map(m, VF);
     build(m, TG_REL_FE);
-    vtkIdType *out = export_values(m, TG_REL_FE);
+    vtkIdType *out = em.identity && fm.identity
+                         ? export_values(m, TG_REL_FE)
+                         : upload_values(renumbered(fetch(m, TG_REL_FE), &fm, &em, 2, 1));
     finish(free_transients);
     return out;
 }
 
-std::unique_ptr<FE_Data> make_FE_GPU(const VF_Data &, const VE_Data &, const vtkIdType,
+std::unique_ptr<FE_Data> make_FE_GPU(const VF_Data &VF, const VE_Data &VE, const vtkIdType,
                                      const vtkIdType n_edges_, const vtkIdType n_faces_,
                                      const bool free_transients) {
     tg_mesh *m = current("make_FE_GPU");
     expect_count("edges", n_edges(m), n_edges_);
     expect_count("faces", n_faces(m), n_faces_);
+    const IdMap &em = edge_map(m, VE);
+    const IdMap &fm = face_map(m, VF);
     build(m, TG_REL_FE);
-    const HostRel h = fetch(m, TG_REL_FE);
+    HostRel h = fetch(m, TG_REL_FE);
+    if (!em.identity || !fm.identity) {
+        const auto v = renumbered(h, &fm, &em, 2, 1);
+        h.val.assign(v.begin(), v.end());
+    }
     auto out = std::make_unique<FE_Data>((size_t)h.rows);
     for (int64_t f = 0; f < h.rows; f++)
         for (int k = 0; k < 3; k++)
