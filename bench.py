@@ -12,7 +12,8 @@ compulsory bytes 16T + 4V + 2(4(V+1) + 8E) + 12V (TV read, scalars read, VV
 CSR written once and read once, classes written); the per-stage split of the
 step comes from HIP events recorded between the pipeline's launches on the
 stream the kernels run on.  `cpu_baseline` is the OpenMP CPU oracle on the
-same mesh, with the thread count it used.
+same mesh (1 warm-up + median of 5, host CPU model and SMT state stated),
+beside the serial form that follows the reference's own builders.
 """
 from __future__ import annotations
 
@@ -67,6 +68,75 @@ def cpu_threads() -> int:
     except AttributeError:
         n = os.cpu_count() or 1
     return max(1, min(n, int(os.environ.get("OMP_NUM_THREADS", n))))
+
+
+def cpu_info() -> dict:
+    """Host CPU of the run: model name, logical CPUs, threads per core (SMT)."""
+    model, siblings, cores = None, None, None
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name" and model is None:
+                model = v
+            elif k == "siblings" and siblings is None:
+                siblings = int(v)
+            elif k == "cpu cores" and cores is None:
+                cores = int(v)
+    except OSError:
+        pass
+    smt = None
+    try:
+        smt = Path("/sys/devices/system/cpu/smt/active").read_text().strip() == "1"
+    except OSError:
+        pass
+    tpc = siblings // cores if siblings and cores else None
+    return {"model": model, "logical_cpus": os.cpu_count(), "threads_per_core": tpc,
+            "smt_active": smt if smt is not None else (tpc > 1 if tpc else None),
+            "affinity_cpus": cpu_threads()}
+
+
+def median_time(fn, reps: int = 5) -> tuple:
+    """One warm-up call, then the median of `reps` timed calls (steady wall
+    clock); returns (median seconds, last result)."""
+    import statistics
+
+    res = fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        res = fn()
+        ts.append(time.perf_counter() - t0)
+    return statistics.median(ts), res
+
+
+def cpu_baselines(t, mesh, threads: int) -> dict:
+    """The CPU restatement of the path (oracle/oracle.c), timed on this host:
+    cpu_omp, the OpenMP form, on the full workload, and cpu_faithful, the
+    serial form that follows the reference's find-based builders
+    (src/cpu_extraction.cpp:259-289) plus the critPoints rule, on a bounded
+    sample of the same workload (Kuhn 48^3, same field), each 1 warm-up +
+    median of 5."""
+    import numpy as np
+    from oracle import binding as O
+
+    T, V = mesh.n_cells, mesh.n_points
+    omp_s, (vv, cls, _) = median_time(lambda: O.vv_cp_omp(mesh.cells, V, mesh.scalars, threads))
+    small = t.kuhn(48, seed=1234, scalar="gaussians")
+    sc64 = small.scalars.astype(np.float64)
+
+    def faithful():
+        VV = O.vv(small.cells, small.n_points)
+        return O.critpoints(VV, sc64, small.n_points)
+
+    ser_s, _ = median_time(faithful)
+    return {"omp": {"value": round(T / omp_s, 1), "unit": "tets/s", "cores": threads,
+                    "seconds_median": round(omp_s, 3), "median_of": 5,
+                    "sample": f"full workload ({T} tets)"},
+            "faithful": {"value": round(small.n_cells / ser_s, 1), "unit": "tets/s", "cores": 1,
+                         "seconds_median": round(ser_s, 3), "median_of": 5,
+                         "sample": f"Kuhn 48^3 with the same field ({small.n_cells} tets)"},
+            "cpu": cpu_info(), "_ref": (vv, cls)}
 
 
 def copy_calibration(torch, mib: int, reps: int) -> int:
@@ -271,10 +341,17 @@ def bench_small(t, torch, steps: int = 200, check: bool = True) -> dict:
     if check:
         from oracle import binding as O
 
-        VV, ref, _ = O.vv_cp_omp(m.cells, m.n_points, m.scalars, cpu_threads())
+        th = cpu_threads()
+        omp_s, (VV, ref, _) = median_time(lambda: O.vv_cp_omp(m.cells, m.n_points, m.scalars, th))
+        sc64 = m.scalars.astype(np.float64)
+        ser_s, _ = median_time(lambda: O.critpoints(O.vv(m.cells, m.n_points), sc64, m.n_points))
         out["identical_to_oracle"] = bool(np.array_equal(VV.off, off)
                                           and np.array_equal(VV.val[:, 0], vals[:, 0])
                                           and np.array_equal(ref, cls))
+        out["cpu_omp"] = {"tets_per_s": round(m.n_cells / omp_s, 1), "cores": th,
+                          "seconds_median": round(omp_s, 4), "median_of": 5}
+        out["cpu_faithful"] = {"tets_per_s": round(m.n_cells / ser_s, 1), "cores": 1,
+                               "seconds_median": round(ser_s, 4), "median_of": 5}
     log(f"small: {out}")
     return out
 
@@ -371,6 +448,10 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=128, help="Kuhn grid side")
     ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--variant", type=int, default=0,
+                    help="kernel form of the hot path (0: the measured tuning table)")
+    ap.add_argument("--bv-log", type=int, default=0,
+                    help="bucketed forms: 2^k vertices per bucket (7 or 8; 0: tuning table)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--batch-meshes", type=int, default=4096,
                     help="meshes in the batching measurement (0: skip)")
@@ -426,6 +507,10 @@ def main() -> int:
     # timing events are recorded on it
     stream = torch.cuda.Stream()
     d = t.Mesh(mesh.cells, V, mesh.scalars, stream=stream.cuda_stream)
+    if args.variant:
+        d.set_config(variant=args.variant)
+    if args.bv_log:
+        d.set_bucket(args.bv_log, 8192)
 
     log(f"workload {workload}: {T} tets, {V} vertices")
     for _ in range(max(args.warmup, 1)):
@@ -496,18 +581,20 @@ def main() -> int:
     log(f"{ms:.4f} ms per step")
     cpu = None
     if not args.no_cpu:
-        from oracle import binding as O
-
         threads = cpu_threads()
-        ref_vv, ref_cls, secs = O.vv_cp_omp(mesh.cells, V, mesh.scalars, threads)
+        cb = cpu_baselines(t, mesh, threads)
+        ref_vv, ref_cls = cb.pop("_ref")
         off, vals = d.relation("VV")
         same = bool(np.array_equal(off, ref_vv.off) and np.array_equal(vals[:, 0], ref_vv.val[:, 0])
                     and np.array_equal(d.classes(), ref_cls))
-        cpu = {"value": round(T / secs, 1), "unit": "tets/s", "cores": threads,
-               "kind": "port",
-               "sample": f"full workload ({T} tets), one pass of the OpenMP oracle "
-                         f"(VV + critical points), {secs:.2f} s",
+        omp = cb["omp"]
+        cpu = {"value": omp["value"], "unit": "tets/s", "cores": threads, "kind": "port",
+               "sample": f"full workload ({T} tets): the OpenMP oracle (VV + critical points), "
+                         f"1 warm-up + median of 5, {omp['seconds_median']} s",
+               "cpu_omp": omp, "cpu_faithful": cb["faithful"], "host": cb["cpu"],
                "gpu_identical_to_oracle": same}
+        log(f"cpu baselines: omp {omp['value']:.3g} tets/s ({threads} threads), "
+            f"faithful {cb['faithful']['value']:.3g} tets/s (1 thread)")
     d.close()
 
     # end to end from host buffers: upload TV + scalars, run, copy VV + classes back

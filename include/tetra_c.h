@@ -91,8 +91,10 @@ int tg_mesh_counts(tg_mesh *m, int64_t *n_points, int64_t *n_cells,
  * 0 automatic, 1 vertex-lane form (vertex -> tet lists, one lane per
  * vertex), 2 bucketed form (corners routed to 128/256-vertex buckets, one
  * workgroup per bucket, shared hash tables), 3 bucketed owner form (the
- * bucket's records grouped by vertex in LDS, one lane per vertex).  lane_cap (32 or 64) and lane_block (64, 128 or
- * 256) shape the vertex-lane form.  Chosen by the autotuner per size class;
+ * bucket's records grouped by vertex in LDS, one lane per vertex), 4
+ * bucketed form with the lean insert (record by record, linear CAS probing).
+ * lane_cap (32 or 64) and lane_block (64, 128 or 256) shape the vertex-lane
+ * form.  Chosen by the autotuner per size class;
  * every supported setting gives identical results. */
 int tg_mesh_set_config(tg_mesh *m, int32_t variant, int32_t lane_cap,
                        int32_t lane_block);
@@ -117,7 +119,7 @@ int tg_mesh_routing(tg_mesh *m, int32_t *route);
  * counterpart in the reference, which has fixed launch shapes. */
 typedef struct tg_tune_entry {
     int64_t max_cells;
-    int32_t variant;     /* 1 vertex-lane, 2 bucketed, 3 bucketed owner form */
+    int32_t variant;     /* 1 vertex-lane, 2 bucketed, 3 bucketed owner form, 4 lean */
     int32_t lane_cap;    /* 32 or 64 */
     int32_t lane_block;  /* 64, 128 or 256 */
     int32_t bv_log;      /* 7 or 8 */

@@ -56,6 +56,12 @@ void die(const char *what) {
     std::abort();
 }
 
+// EMU_YIELD=k: every k-th atomic operation hands the CPU to the next work-item
+// of the block, so lock-free code sees other lanes' updates between its own
+// read and write (lost compare-and-swaps, retried probes).
+int g_yield_every = -1;
+uint64_t g_atomics = 0;
+
 void to_sched(State st) {
     g_f[g_cur].st = st;
 #ifdef EMU_ASAN
@@ -110,12 +116,14 @@ void run_block() {
             __sanitizer_finish_switch_fiber(g_fake, nullptr, nullptr);
 #endif
         }
-        int live = 0, at_block = 0;
+        int live = 0, at_block = 0, runnable = 0;
         for (int i = 0; i < n; i++) {
             live += g_f[i].st != DONE;
             at_block += g_f[i].st == WAIT_BLOCK;
+            runnable += g_f[i].st == RUN;
         }
         if (!live) break;
+        if (runnable) continue;  // work-items that yielded run before any barrier opens
         bool released = false;
         if (at_block == live) {
             for (int i = 0; i < n; i++)
@@ -179,6 +187,15 @@ void exec_grid(const char *name, dim3 grid, dim3 block, const std::function<void
 }  // namespace
 
 int lane() { return g_cur & 63; }
+
+void atomic_point() {
+    if (g_yield_every < 0) {
+        const char *e = std::getenv("EMU_YIELD");
+        g_yield_every = e ? std::atoi(e) : 0;
+    }
+    if (g_yield_every > 0 && g_cur >= 0 && ++g_atomics % (uint64_t)g_yield_every == 0)
+        to_sched(RUN);
+}
 
 void syncthreads() { to_sched(WAIT_BLOCK); }
 

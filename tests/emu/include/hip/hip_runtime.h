@@ -57,6 +57,7 @@ int syncthreads_and(int p);
 uint64_t wave_ballot(int p);
 uint64_t wave_shfl_bits(uint64_t bits, int src_lane);
 int lane();
+void atomic_point();  // may hand the CPU to another work-item (EMU_YIELD)
 }  // namespace emu
 
 #define threadIdx (::emu::g_threadIdx)
@@ -138,21 +139,21 @@ inline typename std::common_type<A, B>::type max(A a, B b) {
 // Work-items of one workgroup never run at the same time, so a plain
 // read-modify-write is atomic with respect to the others.
 template <typename T, typename U>
-inline T atomicAdd(T *p, U v) { T o = *p; *p = (T)(o + (T)v); return o; }
+inline T atomicAdd(T *p, U v) { emu::atomic_point(); T o = *p; *p = (T)(o + (T)v); return o; }
 template <typename T, typename U>
-inline T atomicSub(T *p, U v) { T o = *p; *p = (T)(o - (T)v); return o; }
+inline T atomicSub(T *p, U v) { emu::atomic_point(); T o = *p; *p = (T)(o - (T)v); return o; }
 template <typename T, typename U>
-inline T atomicOr(T *p, U v) { T o = *p; *p = (T)(o | (T)v); return o; }
+inline T atomicOr(T *p, U v) { emu::atomic_point(); T o = *p; *p = (T)(o | (T)v); return o; }
 template <typename T, typename U>
-inline T atomicAnd(T *p, U v) { T o = *p; *p = (T)(o & (T)v); return o; }
+inline T atomicAnd(T *p, U v) { emu::atomic_point(); T o = *p; *p = (T)(o & (T)v); return o; }
 template <typename T, typename U>
-inline T atomicMax(T *p, U v) { T o = *p; if ((T)v > o) *p = (T)v; return o; }
+inline T atomicMax(T *p, U v) { emu::atomic_point(); T o = *p; if ((T)v > o) *p = (T)v; return o; }
 template <typename T, typename U>
-inline T atomicMin(T *p, U v) { T o = *p; if ((T)v < o) *p = (T)v; return o; }
+inline T atomicMin(T *p, U v) { emu::atomic_point(); T o = *p; if ((T)v < o) *p = (T)v; return o; }
 template <typename T, typename U>
-inline T atomicExch(T *p, U v) { T o = *p; *p = (T)v; return o; }
+inline T atomicExch(T *p, U v) { emu::atomic_point(); T o = *p; *p = (T)v; return o; }
 template <typename T, typename U, typename W>
-inline T atomicCAS(T *p, U cmp, W v) { T o = *p; if (o == (T)cmp) *p = (T)v; return o; }
+inline T atomicCAS(T *p, U cmp, W v) { emu::atomic_point(); T o = *p; if (o == (T)cmp) *p = (T)v; return o; }
 
 // ---- runtime API ----------------------------------------------------------
 typedef int hipError_t;

@@ -147,12 +147,17 @@ CASES = {"noalloc": case_noalloc, "after_hot_path": case_after_hot_path, "all_re
 
 
 def main() -> int:
+    import os
+
     name = sys.argv[1]
+    if name.endswith("_y"):  # preempt work-items at every third atomic (lost CAS races)
+        os.environ["EMU_YIELD"] = "3"
+        name = name[:-2]
     args = [int(a) for a in sys.argv[2:]]
     use_emulator()
     t.init(0)
     CASES[name](*args)
-    print(f"ok {name} {' '.join(sys.argv[2:])}", flush=True)
+    print(f"ok {sys.argv[1]} {' '.join(sys.argv[2:])}", flush=True)
     return 0
 
 

@@ -33,7 +33,10 @@ def emu_env():
 
 
 CASES = [
-    ("vv_cp", 5, 6, 1), ("vv_cp", 5, 6, 2), ("vv_cp", 5, 6, 3), ("vv_cp", 7, 9, 2),
+    ("vv_cp", 5, 6, 1), ("vv_cp", 5, 6, 2), ("vv_cp", 5, 6, 3), ("vv_cp", 5, 6, 4),
+    ("vv_cp", 7, 9, 2),
+    # work-items preempted at atomics: compare-and-swap races in the bucket inserts
+    ("vv_cp_y", 6, 7, 2), ("vv_cp_y", 6, 7, 4), ("vv_cp_y", 6, 7, 3),
     ("routes", 4, 1, 7), ("routes", 4, 2, 8),
     ("all_relations", 3, 4, 0), ("all_relations", 3, 4, 1),
     ("hubs", 11, 40), ("soup", 2, 60, 200),

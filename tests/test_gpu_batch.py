@@ -41,7 +41,7 @@ def check_batch(b, meshes):
         assert np.array_equal(off, srt.off) and np.array_equal(vals, srt.val[:, 0]), f"mesh {i} VV"
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
 def test_batch_matches_per_mesh_oracle(gpu, variant):
     meshes = mixed_meshes(1, 40)
     print(f"batch variant={variant} meshes={len(meshes)}", flush=True)

@@ -11,10 +11,12 @@ from oracle import binding as O
 pytestmark = pytest.mark.gpu
 
 
-# (variant, routing): vertex-lane form; bucketed form and bucketed owner form,
-# each with one-level and two-level routing of the corner records
-VARIANTS = [(1, 0), (2, 1), (2, 2), (3, 1), (3, 2)]
-IDS = ["lane", "bucket-1level", "bucket-2level", "owner-1level", "owner-2level"]
+# (variant, routing): vertex-lane form; bucketed form, bucketed owner form and
+# bucketed form with the lean insert, each with one-level and two-level
+# routing of the corner records
+VARIANTS = [(1, 0), (2, 1), (2, 2), (3, 1), (3, 2), (4, 1), (4, 2)]
+IDS = ["lane", "bucket-1level", "bucket-2level", "owner-1level", "owner-2level",
+       "lean-1level", "lean-2level"]
 BUCKETED, BUCKETED_IDS = VARIANTS[1:], IDS[1:]
 
 

@@ -22,7 +22,7 @@ def test_install_and_read_back(tmp_path):
     assert tuning.current() == []
 
 
-@pytest.mark.parametrize("bad", [{"variant": 4}, {"variant": 0}, {"lane_cap": 48}, {"bv_log": 9},
+@pytest.mark.parametrize("bad", [{"variant": 5}, {"variant": 0}, {"lane_cap": 48}, {"bv_log": 9},
                                  {"tets_per_chunk": 100}, {"lane_cap": 64, "lane_block": 256}])
 def test_invalid_entries_rejected(bad):
     e = {"max_cells": 10, "variant": 2, "lane_cap": 32, "lane_block": 128, "bv_log": 7,

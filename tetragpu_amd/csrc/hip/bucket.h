@@ -15,6 +15,8 @@ struct BucketPlan {
     int64_t tpc = 8192;   // tets per chunk
     int32_t slices = 1;   // slices per coarse bin in the second level
     bool owner = false;   // bucket kernel: owner form (records grouped by vertex)
+    int ins = 0;          // bucket kernel insert: 0 batched snapshots, 1 lean lock-step
+                          // (shared tables in both)
 };
 
 // Routing of the corner records to buckets.

@@ -275,7 +275,7 @@ int tg_mesh_stream(tg_mesh *m, void **stream) {
 
 int tg_mesh_set_config(tg_mesh *m, int32_t variant, int32_t lane_cap, int32_t lane_block) {
     if (!m) return fail(TG_ERR_ARG, "NULL mesh");
-    if (variant < 0 || variant > 3) return fail(TG_ERR_ARG, "variant must be 0, 1, 2 or 3");
+    if (variant < 0 || variant > 4) return fail(TG_ERR_ARG, "variant must be 0 to 4");
     if (lane_cap != 32 && lane_cap != 64) return fail(TG_ERR_ARG, "lane_cap must be 32 or 64");
     if (lane_block != 64 && lane_block != 128 && lane_block != 256)
         return fail(TG_ERR_ARG, "lane_block must be 64, 128 or 256");

@@ -43,7 +43,8 @@ struct StageMarks {
 
 // Tunables of the VV + critical point pipeline (picked by the autotuner).
 struct VvCpConfig {
-    int variant = 0;       // 0 auto, 1 vertex-lane form, 2 bucketed, 3 bucketed owner form
+    int variant = 0;       // 0 auto, 1 vertex-lane form, 2 bucketed, 3 bucketed owner form,
+                           // 4 bucketed with the lean insert
     int lane_cap = 32;     // neighbours held per lane (32 or 64)
     int lane_block = 128;  // lanes (vertices) per workgroup
     int corner_block = 256;

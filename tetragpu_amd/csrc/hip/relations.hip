@@ -127,7 +127,7 @@ std::vector<tg_tune_entry> g_tune;  // ascending max_cells
 int set_tuning(const tg_tune_entry *e, int32_t n) {
     std::vector<tg_tune_entry> t(e, e + n);
     for (const auto &x : t)
-        if (x.variant < 1 || x.variant > 3 || (x.lane_cap != 32 && x.lane_cap != 64) ||
+        if (x.variant < 1 || x.variant > 4 || (x.lane_cap != 32 && x.lane_cap != 64) ||
             (x.lane_block != 64 && x.lane_block != 128 && x.lane_block != 256) ||
             (x.lane_cap == 64 && x.lane_block == 256) || (x.bv_log != 7 && x.bv_log != 8) ||
             x.tets_per_chunk < 1024)
@@ -196,7 +196,7 @@ int prepare_buckets(tg_mesh &m) {
 
 int hot_variant(const tg_mesh &m) {
     if (m.cfg.variant == 1 || !m.bplan.ok) return 1;
-    return m.cfg.variant == 3 ? 3 : 2;
+    return m.cfg.variant == 3 || m.cfg.variant == 4 ? m.cfg.variant : 2;
 }
 
 static BucketBuffers bucket_buffers(tg_mesh &m) {
@@ -223,6 +223,7 @@ static int run_vv(tg_mesh &m, bool classify) {
         if ((rc = prepare_buckets(m))) return rc;
         BucketBuffers k = bucket_buffers(m);
         k.plan.owner = hot_variant(m) == 3;
+        k.plan.ins = hot_variant(m) == 4 ? 1 : 0;
         if ((rc = run_vv_cp_bucketed(k, classify, m.stream))) return rc;
     } else {
         if ((rc = prepare_vt(m))) return rc;

@@ -619,7 +619,48 @@ __device__ __forceinline__ void sort32(K (&r)[32]) {
             }
 }
 
-template <int BV_LOG, typename S>
+// Lean insert of one record's three keys into the vertex's 32-slot table:
+// each key probes linearly from its hash slot, claiming a slot by CAS; a
+// slot found holding the key (or claimed by another thread for the same key)
+// places it.  The three probe chains advance together, so a lane costs the
+// longest of its chains, not their sum.  Slots go only from empty (-1) to a
+// key, and every thread follows the same probe sequence for a key, so each
+// key lands in exactly one slot.  A key that meets 32 occupied slots leaves
+// its slot at -1 (the table is full: the vertex goes to the heavy kernel).
+__device__ __forceinline__ void lean_insert3(int32_t *tab, int32_t x, int32_t y, int32_t z,
+                                             int &sx, int &sy, int &sz) {
+    int px = (int)key_hash(x), py = (int)key_hash(y), pz = (int)key_hash(z);
+    bool dx = false, dy = false, dz = false;
+#pragma unroll 1
+    for (int step = 0; step < SLOTS; step++) {
+        if (!dx) {
+            const int32_t o = atomicCAS(tab + px, -1, x);
+            dx = o == -1 || o == x;
+            if (!dx) px = (px + 1) & (SLOTS - 1);
+        }
+        if (!dy) {
+            const int32_t o = atomicCAS(tab + py, -1, y);
+            dy = o == -1 || o == y;
+            if (!dy) py = (py + 1) & (SLOTS - 1);
+        }
+        if (!dz) {
+            const int32_t o = atomicCAS(tab + pz, -1, z);
+            dz = o == -1 || o == z;
+            if (!dz) pz = (pz + 1) & (SLOTS - 1);
+        }
+        if (dx && dy && dz) break;
+    }
+    sx = dx ? px : -1;
+    sy = dy ? py : -1;
+    sz = dz ? pz : -1;
+}
+
+// INS selects phase 1: 0 batches of four records per thread with group
+// snapshots; 1 record by record with lean_insert3, four records per thread
+// in flight, the wave in lock-step per record (about 40 % fewer VALU
+// instructions per key; same time on MI355X, where the insert is bound by the
+// dependent LDS round trips of the probe chains, not by instruction count).
+template <int BV_LOG, typename S, int INS>
 __global__ __launch_bounds__(2 << BV_LOG) void k_bkt_vv_cp(
     const Rec rec, const int32_t *__restrict__ h2s, int32_t nslice,
     const S *__restrict__ scal, int32_t V, int classify, int32_t *__restrict__ scratch,
@@ -662,6 +703,52 @@ __global__ __launch_bounds__(2 << BV_LOG) void k_bkt_vv_cp(
     }
     __syncthreads();
 
+    if constexpr (INS == 1) {
+        // four records per thread in flight: record i + k * NT sits in ring
+        // slot k; a slot is refilled with the record four strides ahead as
+        // soon as its record is taken
+        constexpr int DEPTH = 4;
+        uint32_t q[DEPTH][3];
+#pragma unroll
+        for (int k = 0; k < DEPTH; k++) {
+            const int32_t i = cbeg + tid + k * NT;
+            q[k][0] = 0xFFFFFFFFu;
+            if (i < cend) {
+                q[k][0] = rec.w0[i];
+                q[k][1] = rec.w1[i];
+                q[k][2] = rec.w2[i];
+            }
+        }
+#pragma unroll 1
+        for (int32_t i0 = cbeg + tid; i0 < cend; i0 += DEPTH * NT) {
+#pragma unroll
+            for (int k = 0; k < DEPTH; k++) {
+                const uint32_t w0 = q[k][0], w1 = q[k][1], w2 = q[k][2];
+                const int32_t nx = i0 + (k + DEPTH) * NT;
+                q[k][0] = 0xFFFFFFFFu;
+                if (nx < cend) {
+                    q[k][0] = rec.w0[nx];
+                    q[k][1] = rec.w1[nx];
+                    q[k][2] = rec.w2[nx];
+                }
+                if (w0 == 0xFFFFFFFFu) continue;
+                uint32_t v, x, y, z;
+                rec_unpack(w0, w1, w2, v, x, y, z);
+                const int vl = (int)(v & (BV - 1));
+                int a, b2, c;
+                lean_insert3(key + vl * KS, (int32_t)x, (int32_t)y, (int32_t)z, a, b2, c);
+                atomicAdd(&ccount[vl], 1);
+                if ((a | b2 | c) < 0) {
+                    ovf[vl] = 1;
+                } else {
+                    uint32_t *av = adj + vl * AS;
+                    atomicOr(&av[a], (1u << b2) | (1u << c));
+                    atomicOr(&av[b2], (1u << a) | (1u << c));
+                    atomicOr(&av[c], (1u << a) | (1u << b2));
+                }
+            }
+        }
+    } else
     // phase 1: neighbour sets and shared-tet link adjacency.  For a batch of
     // keys: all group reads are issued together; a key found in its group
     // snapshot is placed, otherwise it claims the snapshot's first empty
@@ -1490,16 +1577,19 @@ int launch_bucketed_typed(tg::BucketBuffers &k, bool classify, hipStream_t s) {
     int32_t nsl = 1;
     int rc = route_records<false>(k, s, ranges, nsl);
     if (rc) return rc;
-#define TG_BKT(KERNEL, BVL)                                                                  \
-    hipLaunchKernelGGL((KERNEL<BVL, S>), dim3(p.nb), dim3(2 << BVL), 0, s, r2, ranges, nsl,   \
+#define TG_BKT(KERNEL)                                                                       \
+    hipLaunchKernelGGL(KERNEL, dim3(p.nb), dim3(p.bv_log == 7 ? 256 : 512), 0, s, r2, ranges, nsl, \
                        scal, V, cls, b.scratch, b.deg, k.rowsrc, b.classes, b.heavy, b.fixd,   \
                        b.rowdesc, b.counters)
     if (p.owner) {
-        if (p.bv_log == 7) TG_BKT(k_bkt_own, 7);
-        else TG_BKT(k_bkt_own, 8);
+        if (p.bv_log == 7) TG_BKT((k_bkt_own<7, S>));
+        else TG_BKT((k_bkt_own<8, S>));
+    } else if (p.ins == 1) {
+        if (p.bv_log == 7) TG_BKT((k_bkt_vv_cp<7, S, 1>));
+        else TG_BKT((k_bkt_vv_cp<8, S, 1>));
     } else {
-        if (p.bv_log == 7) TG_BKT(k_bkt_vv_cp, 7);
-        else TG_BKT(k_bkt_vv_cp, 8);
+        if (p.bv_log == 7) TG_BKT((k_bkt_vv_cp<7, S, 0>));
+        else TG_BKT((k_bkt_vv_cp<8, S, 0>));
     }
 #undef TG_BKT
     TG_LAUNCH_CHECK("k_bkt_vv_cp");
@@ -1535,7 +1625,7 @@ BucketPlan plan_buckets(int64_t n_cells, int64_t n_points, int bv_log, int64_t t
     p.nb = (int32_t)nb;
     p.fs = std::max(0, ceil_log2(nb) - 8);  // coarse bins <= 256
     p.nc = (int32_t)((nb + (1 << p.fs) - 1) >> p.fs);
-    p.ok = V > 0 && V <= (int64_t)M24 + 1 && (1 << p.fs) <= MAX_FINE && p.nc <= MAX_COARSE;
+    p.ok = V > 0 && V <= (int64_t)M24 && (1 << p.fs) <= MAX_FINE && p.nc <= MAX_COARSE;
     p.tpc = std::max<int64_t>(1024, tpc);
     p.nchunks = (int32_t)std::max<int64_t>(1, (T + p.tpc - 1) / p.tpc);
     // about 2048 level-2 workgroups: several rounds of one resident block per CU
