@@ -35,11 +35,34 @@ def bytes_alg(T: int, V: int, E: int) -> int:
     return 16 * T + 4 * V + 2 * (4 * (V + 1) + 8 * E) + 12 * V
 
 
-def load_valu(workload: str):
-    p = ROOT / "profiles" / f"valu_{workload}.json"
+def kernel_digest() -> str:
+    """Digest of the HIP kernel sources: a counter profile applies to the
+    build it was measured on, and is reported only while this matches."""
+    import hashlib
+
+    h = hashlib.sha256()
+    for p in sorted((ROOT / "tetragpu_amd" / "csrc" / "hip").iterdir()):
+        if p.suffix in (".hip", ".h"):
+            h.update(p.name.encode())
+            h.update(p.read_bytes())
+    return h.hexdigest()[:16]
+
+
+def load_profile(kind: str, workload: str):
+    """profiles/<kind>_<workload>.json if it was measured on these kernel
+    sources, else None."""
+    p = ROOT / "profiles" / f"{kind}_{workload}.json"
     if not p.exists():
         return None
-    return json.loads(p.read_text())
+    try:
+        d = json.loads(p.read_text())
+    except (OSError, ValueError):
+        return None
+    return d if d.get("kernel_digest") == kernel_digest() else None
+
+
+def load_valu(workload: str):
+    return load_profile("valu", workload)
 
 
 def valu_busy(workload: str, dom_ms: float):
@@ -53,13 +76,126 @@ def valu_busy(workload: str, dom_ms: float):
 
 
 def load_traffic(workload: str):
-    p = ROOT / "profiles" / f"traffic_{workload}.json"
-    if not p.exists():
-        return None
-    try:
-        return json.loads(p.read_text())
-    except (OSError, ValueError):
-        return None
+    return load_profile("traffic", workload)
+
+
+# ---- rocprofv3 summaries (profiles/) ---------------------------------------------
+PROFILE_RUN = ("python3 $R/bench.py --steps 2 --warmup 1 --hot-path-only")
+
+
+def short_kernel(name: str) -> str:
+    import re
+
+    m = re.search(r"(k_\w+)(<[^>]*>)?", name)
+    return (m.group(1) + (m.group(2) or "")) if m else name[:48]
+
+
+def read_counters(path: Path) -> dict:
+    """{kernel: {counter: [value per dispatch]}} from run_counter_collection.csv."""
+    import csv
+    from collections import defaultdict
+
+    out = defaultdict(lambda: defaultdict(dict))
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            out[short_kernel(r["Kernel_Name"])][r["Counter_Name"]][int(r["Dispatch_Id"])] = \
+                float(r["Counter_Value"])
+    return {k: {c: list(v.values()) for c, v in d.items()} for k, d in out.items()}
+
+
+def summarize(d: Path, workload: str) -> int:
+    """Write profiles/*_<workload>.{json,md} from a directory of rocprofv3
+    outputs made by the commands that the written files list:
+    bench_FETCH_SIZE/ bench_WRITE_SIZE/ copy_FETCH_SIZE/ copy_WRITE_SIZE/
+    (traffic), ktrace/ (kernel trace), pmc_valu/ (instruction counts of the
+    bucket kernel, which bench.py turns into its VALU-busy share)."""
+    import csv
+
+    prof = ROOT / "profiles"
+    dig = kernel_digest()
+    runs = None
+
+    def n_runs(c):
+        return max(len(v) for k, x in c.items() if k.startswith("k_bkt_vv_cp") for v in x.values())
+
+    if all((d / f"{w}_{c}" / "run_counter_collection.csv").exists()
+           for w in ("bench", "copy") for c in ("FETCH_SIZE", "WRITE_SIZE")):
+        scale, per = {}, {}
+        for c in ("FETCH_SIZE", "WRITE_SIZE"):
+            cp = read_counters(d / f"copy_{c}" / "run_counter_collection.csv")
+            vals = [v for k, x in cp.items() for v in x.get(c, [])]
+            # a 1 GiB copy reads and writes 1 GiB: 1048576 KiB per dispatch
+            scale[c] = 1048576.0 / (sum(vals) / len(vals))
+            bc = read_counters(d / f"bench_{c}" / "run_counter_collection.csv")
+            runs = n_runs(bc)
+            per[c] = {k: sum(x.get(c, [])) * 1024 * scale[c] / runs for k, x in bc.items()}
+        kernels = sorted(set(per["FETCH_SIZE"]) | set(per["WRITE_SIZE"]),
+                         key=lambda k: -(per["FETCH_SIZE"].get(k, 0) + per["WRITE_SIZE"].get(k, 0)))
+        dom = next(k for k in kernels if k.startswith("k_bkt_vv_cp"))
+        fetch = sum(per["FETCH_SIZE"].values())
+        write = sum(per["WRITE_SIZE"].values())
+        j = {"workload": workload, "kernel_digest": dig, "hot_path_runs": runs,
+             "bytes_per_step": round(fetch + write), "fetch_bytes_per_step": round(fetch),
+             "write_bytes_per_step": round(write),
+             "fetch_scale": round(scale["FETCH_SIZE"], 4),
+             "write_scale": round(scale["WRITE_SIZE"], 4), "dominant_kernel": "k_bkt_vv_cp",
+             "dominant_kernel_bytes": round(per["FETCH_SIZE"].get(dom, 0)
+                                            + per["WRITE_SIZE"].get(dom, 0)),
+             "kernels": {k: {"fetch": round(per["FETCH_SIZE"].get(k, 0)),
+                             "write": round(per["WRITE_SIZE"].get(k, 0))} for k in kernels}}
+        (prof / f"traffic_{workload}.json").write_text(json.dumps(j, indent=1) + "\n")
+        lines = [f"# HBM traffic per step, {workload} (VV + critical points), MI355X", "",
+                 "Commands (repository root `$R`, after `cd /tmp && export TMPDIR=/tmp`), "
+                 "then `python3 $R/bench.py --summarize <dir>`:", "", "```"]
+        for c in ("FETCH_SIZE", "WRITE_SIZE"):
+            lines += [f"timeout -s KILL 120 rocprofv3 --pmc {c} --output-format csv "
+                      f"-d <dir>/bench_{c} -o run -- {PROFILE_RUN}",
+                      f"timeout -s KILL 90 rocprofv3 --pmc {c} --output-format csv "
+                      f"-d <dir>/copy_{c} -o run -- python3 $R/bench.py --copy-calibration 1024 "
+                      f"--steps 3"]
+        lines += ["```", "",
+                  f"Calibration: a 1 GiB device copy measured by the same counters gives the "
+                  f"factors FETCH_SIZE x{scale['FETCH_SIZE']:.3f}, WRITE_SIZE "
+                  f"x{scale['WRITE_SIZE']:.3f}.  Per step = counter sum over the run / "
+                  f"{runs} hot-path runs.  Kernel sources digest `{dig}`.", "",
+                  "| kernel (one step) | fetch MB | write MB |", "|---|---:|---:|"]
+        for k in kernels:
+            lines.append(f"| {k} | {per['FETCH_SIZE'].get(k, 0) / 1e6:.1f} | "
+                         f"{per['WRITE_SIZE'].get(k, 0) / 1e6:.1f} |")
+        lines.append(f"| **total** | **{fetch / 1e6:.1f}** | **{write / 1e6:.1f}** |")
+        (prof / f"traffic_{workload}.md").write_text("\n".join(lines) + "\n")
+        log(f"traffic: {(fetch + write) / 1e9:.3f} GB per step")
+
+    if (d / "ktrace" / "run_kernel_stats.csv").exists():
+        with open(d / "ktrace" / "run_kernel_stats.csv") as f:
+            rows = list(csv.DictReader(f))
+        lines = [f"# Kernel trace, {workload} (VV + critical points), MI355X", "",
+                 "Command (repository root `$R`, after `cd /tmp && export TMPDIR=/tmp`), "
+                 "then `python3 $R/bench.py --summarize <dir>`:", "", "```",
+                 "timeout -s KILL 200 rocprofv3 --kernel-trace --stats --output-format csv "
+                 "-d <dir>/ktrace -o run -- python3 $R/bench.py --steps 10 --warmup 2 "
+                 "--hot-path-only", "```", "",
+                 f"Kernel sources digest `{dig}`.", "",
+                 "| kernel | calls | avg us | min us | max us | share % |",
+                 "|---|---:|---:|---:|---:|---:|"]
+        for r in rows:
+            lines.append(f"| {short_kernel(r['Name'])} | {r['Calls']} | "
+                         f"{float(r['AverageNs']) / 1e3:.1f} | {float(r['MinNs']) / 1e3:.1f} | "
+                         f"{float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.2f} |")
+        (prof / f"kernels_{workload}.md").write_text("\n".join(lines) + "\n")
+
+    if (d / "pmc_valu" / "run_counter_collection.csv").exists():
+        cc = read_counters(d / "pmc_valu" / "run_counter_collection.csv")
+        ks = {k: {c: sum(v) / len(v) for c, v in x.items()} for k, x in cc.items()
+              if k.startswith("k_bkt_vv_cp")}
+        (prof / f"valu_{workload}.json").write_text(json.dumps(
+            {"workload": workload, "kernel_digest": dig, "clock_hz": 2.4e9, "simds": 1024,
+             "command": "timeout -s KILL 90 rocprofv3 --pmc SQ_INSTS_VALU SQ_INSTS_SALU "
+                        "SQ_INSTS_LDS SQ_WAVES SQ_WAVE_CYCLES SQ_BUSY_CYCLES --output-format csv "
+                        f"-d <dir>/pmc_valu -o run -- {PROFILE_RUN}",
+             "kernels": ks}, indent=1) + "\n")
+        log(f"valu: {list(ks)}")
+    return 0
 
 
 def cpu_threads() -> int:
@@ -225,7 +361,8 @@ def bench_batch(t, torch, n_meshes: int, reps: int, seed: int = 7) -> dict:
 def bench_serve(t, rates, n_requests: int, seed: int = 11) -> dict:
     """Poisson arrivals of VV + critical-point requests (Kuhn grids with 10 %
     splits, T log-uniform in [10k, 5M], Gaussian-mixture scalars) followed by
-    a VT + TT query on the resident mesh; p50/p99 latency per rate."""
+    a VT + TT query on the resident mesh (built on the device, kept there);
+    p50/p99 latency per rate, throughput, and the latency's phases."""
     import threading
     from concurrent.futures import ThreadPoolExecutor
 
@@ -244,6 +381,8 @@ def bench_serve(t, rates, n_requests: int, seed: int = 11) -> dict:
            "rates": []}
     for rate in rates:
         lat1, lat2, cells = [], [], []
+        phases = {k: [] for k in ("queue_ms", "upload_ms", "compute_ms", "followup_ms",
+                                  "client_ms")}
         lock = threading.Lock()
         with Server(slots=4, hbm_budget_bytes=96 << 30, fuse_max_cells=1 << 19,
                     fuse_max_meshes=64) as srv, ThreadPoolExecutor(64) as ex:
@@ -251,13 +390,19 @@ def bench_serve(t, rates, n_requests: int, seed: int = 11) -> dict:
                 m = pool[k]
                 rid = srv.submit(m.cells, m.n_points, m.scalars)
                 r1 = srv.wait(rid)
-                srv.followup(rid, VT | TT, copy=False)
+                got = time.perf_counter()
+                f = srv.followup(rid, VT | TT, copy=False)
                 done = time.perf_counter()
                 srv.release(rid)
                 with lock:
                     lat1.append(r1["total_ms"])
                     lat2.append((done - submitted) * 1e3)
                     cells.append(m.n_cells)
+                    for key in ("queue_ms", "upload_ms", "compute_ms"):
+                        phases[key].append(r1[key])
+                    phases["followup_ms"].append(f["followup_ms"])
+                    # Python submit/wait/return around the server's own clock
+                    phases["client_ms"].append((got - submitted) * 1e3 - r1["total_ms"])
             picks = rng.integers(0, pool_n, n_requests)
             gaps = rng.exponential(1.0 / rate, n_requests)
             start = time.perf_counter()
@@ -273,11 +418,18 @@ def bench_serve(t, rates, n_requests: int, seed: int = 11) -> dict:
                 f.result()
             wall = time.perf_counter() - start
             groups = srv.stats()["groups"]
-        log(f"serve: rate {rate}: p50 {pct(lat1, 50)} ms p99 {pct(lat1, 99)} ms")
+        thr = n_requests / wall
+        log(f"serve: rate {rate}: p50 {pct(lat1, 50)} ms p99 {pct(lat1, 99)} ms, "
+            f"{thr:.1f} req/s")
         out["rates"].append({"rate": rate, "p50_ms": pct(lat1, 50), "p99_ms": pct(lat1, 99),
                              "p50_ms_with_followup": pct(lat2, 50),
                              "p99_ms_with_followup": pct(lat2, 99),
-                             "throughput_req_s": round(n_requests / wall, 1),
+                             "throughput_req_s": round(thr, 1),
+                             # offered load not kept up with: the queue grows
+                             # during the run and p99 measures that growth
+                             "saturated": bool(thr < 0.95 * rate),
+                             "phase_p50_ms": {k: pct(v, 50) for k, v in phases.items()},
+                             "phase_p99_ms": {k: pct(v, 99) for k, v in phases.items()},
                              "tets_per_s": round(sum(cells) / wall, 1), "launches": groups})
     return out
 
@@ -415,14 +567,79 @@ def bench_irregular(t, torch, steps: int, seed: int = 5, check: bool = True) -> 
     return out
 
 
+def bench_translation(seed: int) -> dict:
+    """Device->host transfer and host translation of the drop-in VV builder
+    (make_VV_GPU: CSR copied back, then one std::vector per vertex), timed by
+    the builder's own timers in the command-line driver, beside the builder's
+    whole time (upload of TV excluded, build included).  The reference reports
+    the same three intervals for its VV builder (README.md:115-123)."""
+    import re
+    import subprocess
+
+    exe = Path(__file__).resolve().parent / "tetragpu_amd" / "lib" / "tetra_main"
+    out = []
+    for n in (24, 128):
+        cmd = [str(exe), "-i", f"kuhn:{n}:{seed}:gaussians", "--no_validate", "-d", "1",
+               "--build_VV", "-e", "/dev/null"]
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+        if r.returncode != 0:
+            raise RuntimeError(f"{' '.join(cmd)} failed: {r.stderr[-500:]}")
+
+        def grab(pat):
+            m = re.search(pat, r.stdout)
+            return float(m.group(1)) if m else None
+        row = {"workload": f"kuhn{n}_gaussians", "tets": 6 * n ** 3,
+               "vv_builder_s": grab(r"\[time\] VV \[GPU\]: ([0-9.eE+-]+) s"),
+               "transfer_s": grab(r"Timer\[VV_GPU\] GPU Device->Host transfer: ([0-9.]+)"),
+               "translation_s": grab(r"Timer\[VV_GPU\] GPU Device->Host translation: ([0-9.]+)")}
+        out.append(row)
+        log(f"translation: {row}")
+    return {"builder": "make_VV_GPU via tetra_main", "meshes": out}
+
+
+def bench_split(t, torch, mesh, whole, seed: int, div: int = 4) -> dict:
+    """The automatic split (tg_critpoints_split) of the benchmark mesh under
+    a device budget of the resident arrays plus 1/div of the whole mesh's
+    footprint: vertex-id ranges, each classified on the cells that touch the
+    range or its neighbours, must equal the whole-mesh run.  Run on the
+    benchmark mesh (random vertex order: wide halos) and on the same grid
+    with vertices and cells in grid order (the order of most mesh files)."""
+    import numpy as np
+
+    out = []
+    n = round((mesh.n_cells / 6) ** (1 / 3))
+    ordered = t.kuhn(n, seed=seed, permute_vertices=False, permute_cells=False,
+                     scalar="gaussians")
+    with t.Mesh(ordered.cells, ordered.n_points, ordered.scalars) as d:
+        d.vv_critpoints()
+        whole_ordered = d.classes()
+    for name, m, ref in (("random_order", mesh, whole), ("grid_order", ordered, whole_ordered)):
+        T, V = m.n_cells, m.n_points
+        resident = 16 * T + 4 * V + 12 * V + 4 * (T + 1) + 4 * (V + 1) + V + (1 << 20)
+        budget = resident + t.vv_footprint(T, V) // div
+        sc = m.scalars.astype(np.float32)
+        for _ in range(2):
+            cls, st = t.split_classes(m.cells, V, sc, budget_bytes=budget)
+        row = {"order": name, "budget_bytes": budget, "footprint_bytes": st["footprint_bytes"],
+               "parts": st["parts"], "largest_part_cells": st["max_cells"],
+               "cells_over_all_parts": st["cells_total"],
+               "total_s": round(st["total_s"], 3), "localize_s": round(st["localize_s"], 3),
+               "classify_s": round(st["classify_s"], 3),
+               "identical_to_whole_mesh": bool(np.array_equal(cls, ref))}
+        out.append(row)
+        log(f"split: {row}")
+    return {"rule": f"budget = resident arrays + footprint/{div}", "runs": out}
+
+
 def bench_partitioned(t, mesh, whole, n_parts: int = 8) -> dict:
     """The partition loop (src/alg/critPoints.cu:654-1003) on the benchmark
     mesh cut into n_parts slabs of equal z extent.  Each slab is classified on
     the cells that touch it (a one-cell halo), one slab at a time on the GPU,
     so only the largest slab's mesh is resident.  Kuhn grids are flag
     complexes, so the merged classes must equal the whole-mesh run.  Times are
-    host wall clock: localization on the host, then per slab upload + VV +
-    classes + copy back (second of two runs)."""
+    host wall clock: localisation on the device (cell and vertex flags,
+    scans, local ids), then per slab mesh creation + VV + classes + merge on
+    the device (second of two runs)."""
     import numpy as np
 
     z = mesh.positions[:, 2].astype(np.float64)
@@ -455,7 +672,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--batch-meshes", type=int, default=4096,
                     help="meshes in the batching measurement (0: skip)")
-    ap.add_argument("--serve-requests", type=int, default=60,
+    ap.add_argument("--serve-requests", type=int, default=1000,
                     help="requests per arrival rate in the serving measurement (0: skip)")
     ap.add_argument("--rates", type=float, nargs="+", default=[100, 500, 1000, 2000, 5000])
     ap.add_argument("--no-relations", action="store_true",
@@ -470,7 +687,18 @@ def main() -> int:
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (reference kernel for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
+    ap.add_argument("--hot-path-only", action="store_true",
+                    help="only the benchmark mesh's warm-up, timed and staged steps (for "
+                         "counter profiles: every dispatch is the hot path's)")
+    ap.add_argument("--summarize", type=Path, default=None, metavar="DIR",
+                    help="only write profiles/*.json|md from the rocprofv3 outputs in DIR "
+                         "(no GPU)")
     args = ap.parse_args()
+    if args.summarize:
+        return summarize(args.summarize, f"kuhn{args.n}_gaussians")
+    if args.hot_path_only:
+        args.no_cpu = args.no_relations = args.no_irregular = args.no_partitions = True
+        args.batch_meshes = args.serve_requests = 0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the path is not sharded across devices "
               f"(--gpus {args.gpus} requested)", file=sys.stderr)
@@ -597,6 +825,12 @@ def main() -> int:
             f"faithful {cb['faithful']['value']:.3g} tets/s (1 thread)")
     d.close()
 
+    if args.hot_path_only:
+        print(json.dumps({"metric": METRIC, "value": round(T / (ms * 1e-3), 1), "unit": "tets/s",
+                          "ms_per_step": round(ms, 4), "hot_path_only": True,
+                          "hot_path_runs": max(args.warmup, 1) + 2 * args.steps + 1,
+                          "roofline": roofline}))
+        return 0
     # end to end from host buffers: upload TV + scalars, run, copy VV + classes back
     t0 = time.perf_counter()
     d2 = t.Mesh(mesh.cells, V, mesh.scalars, stream=stream.cuda_stream)
@@ -607,8 +841,10 @@ def main() -> int:
     d2.close()
 
     partitioned = None if args.no_partitions else bench_partitioned(t, mesh, whole_cls)
+    split = None if args.no_partitions else bench_split(t, torch, mesh, whole_cls, args.seed)
 
     relations = None if args.no_relations else bench_relations(t, torch, mesh)
+    translation = None if args.no_relations else bench_translation(args.seed)
     small = bench_small(t, torch, check=not args.no_cpu)
     irregular = None if args.no_irregular else bench_irregular(t, torch, max(args.steps // 2, 1),
                                                                 check=not args.no_cpu)
@@ -638,9 +874,11 @@ def main() -> int:
         "end_to_end_ms": round(e2e_ms, 2),
         "end_to_end_note": "host cells/scalars in, VV CSR + classes out, incl. allocation (first use)",
         "relations": relations,
+        "host_translation": translation,
         "small": small,
         "irregular": irregular,
         "partitioned": partitioned,
+        "split": split,
         "batched": batch,
         "serving": serve,
     }

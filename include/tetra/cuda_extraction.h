@@ -93,4 +93,18 @@ std::unique_ptr<VT_Data> make_VT_GPU(const TV_Data &TV, const bool free_transien
 // Release the calling thread's current mesh (what free_transients does).
 void free_transients_GPU();
 
+// Device->host transfer and host translation times of the builders that
+// return host structures.  The reference's builders time the same two
+// intervals with their own timers and print them (src/cuda_extraction.cu:
+// 326-336, 1250-1265: "GPU Device->Host transfer", "GPU Device->Host
+// translation"); here they are recorded per calling thread and printed to
+// stdout in that form only when reporting is on (off by default).
+struct HostTiming {
+    std::string timer;  // "VV_GPU", "TE_GPU", ...
+    std::string label;  // "GPU Device->Host transfer" / "... translation"
+    double seconds;
+};
+void set_host_timing_report(bool on);
+std::vector<HostTiming> take_host_timings();  // this thread's, since the last call
+
 #endif

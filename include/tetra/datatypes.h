@@ -115,6 +115,10 @@ typedef std::vector<std::vector<vtkIdType>> TT_Data;
 typedef struct argstruct {
     int threadNumber = 1, max_VV = -1, n_GPUS = 0, debug = DEBUG_MAX;
     bool no_partitioning = false;
+    // export_classes also writes one "A <vertex> <class>" line per classified
+    // vertex (the short form that the reference's pyclassify.py:51-59 reads;
+    // the reference's own line for it is commented out, critPoints.cu:396-401)
+    bool export_vertices = false;
     std::string fileName = "tiny.vtu", export_ = "", arrayname = "", partitioningname = "",
                 thread_logname = "";
     // bit 0 validate, bits 1..13 TE EV ET TF FV FE FT EF VT TT FF EE VV

@@ -46,16 +46,28 @@ typedef struct tg_request_result {
     double run_ms;              /* start -> first result (incl. submit-time follow-ups) */
     double total_ms;            /* submit -> first result */
     int32_t group_size;         /* requests fused into the launch that served it */
+    /* phases of run_ms (for a fused group, the group's): */
+    double upload_ms;           /* mesh creation: host->device copy, connectivity check */
+    double compute_ms;          /* VV + critical points and the classes' copy back */
+    /* the last follow-up's phases (also submit-time ones): */
+    double followup_ms;         /* building VT / TT on the device */
+    double copy_ms;             /* their copy to host memory (0 without TG_SERVE_COPY) */
 } tg_request_result;
+
+/* Follow-up flag: also copy the relations to host memory (vt_off/vt_val,
+ * tt_off/tt_val).  Without it they are built and stay on the device (counts
+ * in vt_nnz / tt_nnz), ready for the next query. */
+#define TG_SERVE_COPY (1u << 31)
 
 int tg_server_create(tg_server **out, const tg_server_config *cfg);
 /* followups: TG_FLAG(TG_REL_VT) | TG_FLAG(TG_REL_TT) to compute with the
- * first result (counted in its latency). */
+ * first result (counted in its latency), optionally | TG_SERVE_COPY. */
 int tg_server_submit(tg_server *s, const int32_t *cells, int64_t n_cells, int64_t n_points,
                      const float *scalars, uint32_t followups, uint64_t *id);
 /* Block until the request's first result; pointers stay valid until release. */
 int tg_server_wait(tg_server *s, uint64_t id, tg_request_result *out);
-/* Follow-up query on the request's resident relations (VT and/or TT). */
+/* Follow-up query on the request's resident relations (VT and/or TT,
+ * optionally | TG_SERVE_COPY). */
 int tg_server_followup(tg_server *s, uint64_t id, uint32_t flags, tg_request_result *out);
 /* Drop the request; its device memory returns to the budget when every
  * request of its group is released. */

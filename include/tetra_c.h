@@ -249,6 +249,49 @@ int tg_host_free(void *p);
 /* Free a device pointer handed out by the library to the caller. */
 int tg_device_free(void *p);
 
+/* ---- critical points in parts ----------------------------------------
+ * Replaces the device side of the reference's partition loop
+ * (src/alg/critPoints.cu:654-1003, merged at :1438-1491) and adds the split
+ * that its memory handling (:1038-1084, buffers sized per partition) leaves
+ * to the caller.
+ * tg_vv_footprint: device bytes the VV + critical-point path allocates for a
+ * mesh of this size (TV included).
+ * tg_critpoints_split: classes [n_points][3] (host) of cells [n_cells][4]
+ * int32 and scalars [n_points] float32 / float64 (host).
+ *   parts == NULL: the whole mesh on one handle when it fits budget_bytes
+ *   and the handle's 32-bit limits; else vertex-id ranges, each classified
+ *   on the cells that touch the range or its neighbours (results identical
+ *   to the whole mesh; a range whose part does not fit is halved).
+ *   parts [n_points] in 0..n_parts-1: the reference's partitions, largest
+ *   first; partition p uses row p of part_cells [n_parts][n_cells] (1 =
+ *   included) when given, else every cell touching one of its vertices; a
+ *   vertex keeps the classes of its own partition ({0,0,0} if no cell of
+ *   that partition holds it).
+ *   budget_bytes <= 0: 90 % of the device memory free now (the library's
+ *   cached blocks counted as free).
+ * Localisation runs on the device; TV and scalars stay resident.  Returns
+ * TG_ERR_ALLOC when TV and scalars alone, a caller partition or a
+ * one-vertex range do not fit the budget. */
+typedef struct tg_split_stats {
+    int64_t parts;            /* parts classified (with at least one cell) */
+    int64_t max_cells;        /* largest part */
+    int64_t max_points;
+    int64_t cells_total;      /* over all parts (shared cells counted again) */
+    int64_t footprint_bytes;  /* tg_vv_footprint of the whole mesh */
+    int64_t budget_bytes;     /* the budget used */
+    double total_s;           /* wall time of the call */
+    double localize_s;        /* device localisation (flags, scans, local ids) */
+    double classify_s;        /* part meshes: create, VV + classes, merge */
+} tg_split_stats;
+int tg_vv_footprint(int64_t n_cells, int64_t n_points, int64_t *bytes);
+/* The default budget: 90 % of the device memory free now (cached blocks
+ * counted as free). */
+int tg_device_budget(int64_t *bytes);
+int tg_critpoints_split(const int32_t *cells, int64_t n_cells, int64_t n_points,
+                        const void *scalars, int scalar_f64, const int32_t *parts,
+                        int32_t n_parts, const int32_t *part_cells, int64_t budget_bytes,
+                        uint32_t *classes3, tg_split_stats *stats);
+
 /* ---- no-allocation entry point ---------------------------------------
  * No counterpart in the reference: it is for callers that cannot allocate
  * device memory (code under HIP graph capture, tight serving loops,

@@ -4,7 +4,7 @@
 //
 //   dropin_probe cpu <cells.i64> <n_points> [<scalars.f64>]
 //   dropin_probe parse <args...>
-//   dropin_probe export <classes.u32> <n_points> <debug>
+//   dropin_probe export <classes.u32> <n_points> <debug> [parts=<ids.i32>:<n>] [vertices]
 //   dropin_probe load <input> [<arrayname> [<partitioningname>]]
 //   dropin_probe part <cells.i64> <n_points> <scalars.f64> <parts.i32> <cpu|gpu>
 //                [<partcells.i32>]
@@ -136,12 +136,25 @@ int parse_cmd(int argc, char **argv) {
     return 0;
 }
 
-int export_cmd(char **argv) {
+// export <classes.u32> <n_points> <debug> [parts=<ids.i32>:<n_parts>] [vertices]
+int export_cmd(int argc, char **argv) {
     auto cls = read_file<uint32_t>(argv[2]);
     const vtkIdType V = std::atoll(argv[3]);
     TV_Data tv(V, 0);
     runtime_arguments args;
     args.debug = std::atoi(argv[4]);
+    for (int i = 5; i < argc; i++) {
+        const std::string a = argv[i];
+        if (a == "vertices") {
+            args.export_vertices = true;
+        } else if (a.rfind("parts=", 0) == 0) {
+            const size_t colon = a.rfind(':');
+            const auto ids = read_file<int32_t>(a.substr(6, colon - 6).c_str());
+            tv.n_partitions = std::atoi(a.c_str() + colon + 1);
+            tv.partitionIDs = new int[(size_t)V];
+            for (vtkIdType v = 0; v < V; v++) tv.partitionIDs[v] = ids[(size_t)v];
+        }
+    }
     export_classes(cls.data(), tv, args);
     return 0;
 }
@@ -278,7 +291,7 @@ int main(int argc, char **argv) {
     const std::string cmd = argv[1];
     if (cmd == "cpu") return cpu(argc, argv);
     if (cmd == "parse") return parse_cmd(argc, argv);
-    if (cmd == "export") return export_cmd(argv);
+    if (cmd == "export") return export_cmd(argc, argv);
     if (cmd == "load") return load_cmd(argc, argv);
     if (cmd == "part") return part_cmd(argc, argv);
     if (cmd == "gpuids") return gpuids_cmd(argc, argv);

@@ -298,6 +298,11 @@ hipError_t hipMemsetAsync(void *p, int v, size_t bytes, hipStream_t) {
 }
 hipError_t hipStreamCreate(hipStream_t *s) { *s = new emu_stream{1}; return hipSuccess; }
 hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned) { return hipStreamCreate(s); }
+// a fixed 16 GiB "device": budgets in tests are passed explicitly
+hipError_t hipMemGetInfo(size_t *free_bytes, size_t *total_bytes) {
+    *free_bytes = *total_bytes = (size_t)16 << 30;
+    return hipSuccess;
+}
 hipError_t hipStreamDestroy(hipStream_t s) { delete s; return hipSuccess; }
 hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
 hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }

@@ -207,6 +207,7 @@ hipError_t hipMemset(void *p, int v, size_t bytes);
 hipError_t hipMemsetAsync(void *p, int v, size_t bytes, hipStream_t s = nullptr);
 hipError_t hipStreamCreate(hipStream_t *s);
 hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned flags);
+hipError_t hipMemGetInfo(size_t *free_bytes, size_t *total_bytes);
 hipError_t hipStreamDestroy(hipStream_t s);
 hipError_t hipStreamSynchronize(hipStream_t s);
 hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t e, unsigned flags = 0);

@@ -142,7 +142,47 @@ def case_noalloc(seed: int = 6, n1: int = 7, n2: int = 4) -> None:
         assert np.array_equal(cls, cls_ref), "classes"
 
 
-CASES = {"noalloc": case_noalloc, "after_hot_path": case_after_hot_path, "all_relations": case_all_relations,
+def split_budget(T: int, V: int, frac: float) -> int:
+    """A device budget that holds the resident arrays of a split run plus
+    `frac` of the whole mesh's footprint."""
+    resident = 16 * T + 8 * V + 12 * V + 4 * (T + 1) + 4 * (V + 1) + V + (16 << 10) + 2048
+    return resident + int(frac * t.vv_footprint(T, V))
+
+
+def case_split(seed: int = 3, n: int = 4) -> None:
+    """Critical points in parts, localised on the device: vertex-id ranges
+    forced by a small budget equal the whole mesh (a stellar-split mesh with
+    an isolated vertex); caller partitions equal the oracle's partition loop
+    (explicit cell rows, a non-flag link); budgets that cannot hold the
+    resident arrays or a partition fail with TG_ERR_ALLOC."""
+    from oracle import binding as O
+    from test_dropin_cpu import partition_cases
+
+    m = t.kuhn(n, seed=seed, permute_vertices=False, permute_cells=False, split_fraction=0.3,
+               scalar="sines")
+    cells, V = m.cells, m.n_points + 1          # one vertex in no cell
+    s = np.append(m.scalars.astype(np.float64), 0.5)
+    whole = O.critpoints(O.vv(cells, V), s, V)
+    assert whole[-1].tolist() == [0, 0, 4]
+    got, st = t.split_classes(cells, V, s, budget_bytes=split_budget(len(cells), V, 0.6))
+    assert st["parts"] >= 2, st
+    assert np.array_equal(got, whole), ("split", st)
+    assert st["cells_total"] > len(cells)       # halo cells are counted again
+    for name, cells_p, Vp, sp, parts, pc in partition_cases():
+        if name in ("soup_lists", "non_flag"):
+            got, st = t.split_classes(cells_p, Vp, sp, parts=parts, part_cells=pc)
+            assert np.array_equal(got, O.critpoints_partitioned(cells_p, Vp, sp, parts, pc)), name
+    for kw in (dict(budget_bytes=4096),
+               dict(parts=np.zeros(V, np.int32), budget_bytes=split_budget(len(cells), V, 0.2))):
+        try:
+            t.split_classes(cells, V, s, **kw)
+        except _native.NativeError as e:
+            assert e.status == -3, e
+        else:
+            raise AssertionError(f"expected TG_ERR_ALLOC for {list(kw)}")
+
+
+CASES = {"split": case_split, "noalloc": case_noalloc, "after_hot_path": case_after_hot_path, "all_relations": case_all_relations,
          "vv_cp": case_vv_cp, "hubs": case_hubs, "soup": case_soup, "routes": case_routes}
 
 

@@ -155,6 +155,63 @@ def test_export_classes_labels_and_insanity(probe, tmp_path):
     assert "Insanity detected" in r.stderr and " 1 points" in r.stderr  # (1,0,3)
 
 
+def pyclassify_read(text):
+    """The line rules of the reference's pyclassify.py:37-59, restated: lines
+    opening with 'Total' or 'Partition' are skipped; a line holding 'Upper:'
+    and 'Lower:' is the long form (id = third word, class = first class name
+    found in it); any other line ends in its class number, and its vertex id
+    is the second-last word (INSANITY lines are set aside)."""
+    names = ["NULL", "min", "max", "regular", "saddle"]
+    classes, insane = {k: [] for k in range(5)}, []
+    for l in text.splitlines(keepends=True):
+        if l.startswith("Total") or l.startswith("Partition"):
+            continue
+        if "Upper:" in l and "Lower:" in l:
+            vid = int(l.split(" ", 3)[2])
+            for k, n in enumerate(names):
+                if n in l:
+                    classes[k].append(vid)
+                    break
+            continue
+        last = int(l.rstrip().rsplit(" ", 1)[1])
+        if "INSANITY" in l:
+            insane.append(last)
+        else:
+            classes[last].append(int(l.rsplit(" ", 3)[1]))
+    return classes, insane
+
+
+def test_export_partition_detail_and_vertex_lines(probe, tmp_path):
+    # (upper, lower, class) per vertex; partitions 0/1 alternate in pairs
+    cls = np.array([[0, 1, 1], [1, 0, 2], [2, 2, 4], [1, 1, 3], [0, 2, 1], [1, 1, 3],
+                    [0, 0, 0], [3, 2, 4]], np.uint32)
+    parts = np.array([0, 1, 1, 0, 1, 1, 0, 1], np.int32)
+    cp, pp = tmp_path / "c.u32", tmp_path / "p.i32"
+    cls.tofile(cp)
+    parts.tofile(pp)
+    r = run(probe, "export", cp, len(cls), 0, f"parts={pp}:2")
+    detail = [l for l in r.stdout.splitlines() if " has a " in l]
+    # critPoints.cu:402-409: partition 1, class neither void nor regular, in
+    # vertex order, before the totals
+    assert detail == [
+        "Partition 1 has a min for global vertex ID 1 (upper: 1, lower: 0)",
+        "Partition 1 has a saddle for global vertex ID 2 (upper: 2, lower: 2)",
+        "Partition 1 has a max for global vertex ID 4 (upper: 0, lower: 2)",
+        "Partition 1 has a saddle for global vertex ID 7 (upper: 3, lower: 2)"]
+    assert r.stdout.splitlines()[len(detail)].startswith("Total number of minima:")
+    assert not any(l.startswith("A ") for l in r.stdout.splitlines())
+    # without partition ids nothing is partition 1
+    r0 = run(probe, "export", cp, len(cls), 0)
+    assert " has a " not in r0.stdout
+
+    r = run(probe, "export", cp, len(cls), 2, f"parts={pp}:2", "vertices")
+    got, insane = pyclassify_read(r.stdout)
+    assert insane == []
+    want = {k: [v for v in range(len(cls)) if cls[v, 2] == k] for k in range(5)}
+    want[0] = []  # void vertices get no line
+    assert got == want
+
+
 # ---- .vtu reader ----------------------------------------------------------------------
 VTK_TYPES = {np.dtype(np.int32): "Int32", np.dtype(np.int64): "Int64",
              np.dtype(np.float32): "Float32", np.dtype(np.float64): "Float64",

@@ -44,6 +44,7 @@ CASES = [
     ("after_hot_path", 8, 5, 1), ("after_hot_path", 8, 5, 2),
     # the no-allocation entry point: two calls, the second smaller
     ("noalloc", 6, 7, 4),
+    ("split", 3, 4),
 ]
 
 

@@ -52,6 +52,14 @@ def test_all_relations_validated(gpu, spec, tmp_path):
     assert tot["maxima"] == (cls[:, 2] == 1).sum()
     assert tot["saddles"] == (cls[:, 2] == 4).sum()
     assert tot["regular"] == (cls[:, 2] == 3).sum()
+    # each host-returning builder reports its device->host transfer and
+    # translation (src/cuda_extraction.cu:326-336) in the reference's timer form
+    for rel in ("TE", "EV", "ET", "TF", "FV", "FE", "FT", "EF", "VT", "TT", "FF", "EE", "VV"):
+        for what in ("transfer", "translation"):
+            line = f"Timer[{rel}_GPU] GPU Device->Host {what}: "
+            got = [l for l in r.stdout.splitlines() if line in l]
+            assert len(got) == 1, (line, r.stdout[-3000:])
+            assert float(got[0].split(": ")[1]) >= 0
 
 
 def test_vtu_input_default_flags(gpu, tmp_path):
@@ -94,7 +102,7 @@ def test_tetra_main_partitions_from_vtu(gpu, tmp_path):
     write_vtu(path, m.cells, np.zeros((m.n_points, 3)),
               [("f", m.scalars), ("part", (parts + 1).astype(np.int32))], "binary", True)
     out = tmp_path / "classes.txt"
-    r = tetra_main("-i", path, "-a", "f", "-p", "part", "-e", out, "-d", "2")
+    r = tetra_main("-i", path, "-a", "f", "-p", "part", "-e", out, "-d", "2", "--export_vertices")
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     assert "[validate] all checks passed" in r.stderr
     cls = O.critpoints_partitioned(m.cells, m.n_points, m.scalars.astype(np.float64), parts)
@@ -106,6 +114,18 @@ def test_tetra_main_partitions_from_vtu(gpu, tmp_path):
         line = f"Partition {p} number of saddles:"
         got = [int(l.split(":")[1]) for l in text.splitlines() if l.startswith(line)]
         assert got == [int(((cls[:, 2] == 4) & (parts == p)).sum())]
+    # partition 1's non-regular vertices, one line each (critPoints.cu:402-409)
+    names = {1: "max", 2: "min", 4: "saddle"}
+    want = [f"Partition 1 has a {names[int(c)]} for global vertex ID {v} "
+            f"(upper: {cls[v, 0]}, lower: {cls[v, 1]})"
+            for v, c in enumerate(cls[:, 2]) if parts[v] == 1 and c not in (0, 3)]
+    assert want and [l for l in text.splitlines() if " has a " in l] == want
+    # the per-vertex lines read back through pyclassify's rules
+    from test_dropin_cpu import pyclassify_read
+    got, insane = pyclassify_read(text)
+    assert insane == [] and got[0] == []
+    for k in (1, 2, 3, 4):
+        assert got[k] == [int(v) for v in np.flatnonzero(cls[:, 2] == k)]
 
 
 def perm_files(tmp_path, E, F, seed):

@@ -56,11 +56,11 @@ def serve_all(reqs, followups, alone, **cfg):
 
 
 def test_alone_equals_in_the_mix(gpu):
-    from tetragpu_amd.serve import TT, VT
+    from tetragpu_amd.serve import COPY, TT, VT
     reqs = requests(1, 30)
     print(f"serve {len(reqs)} requests", flush=True)
-    alone, _ = serve_all(reqs, VT | TT, True)
-    mixed, stats = serve_all(reqs, VT | TT, False, slots=3, hbm_budget_bytes=8 << 20,
+    alone, _ = serve_all(reqs, VT | TT | COPY, True)
+    mixed, stats = serve_all(reqs, VT | TT | COPY, False, slots=3, hbm_budget_bytes=8 << 20,
                              fuse_max_cells=4000, fuse_max_meshes=8)
     assert stats["groups"] < len(reqs)  # some requests were fused
     assert any(r["group_size"] > 1 for r in mixed)
@@ -89,9 +89,16 @@ def test_followup_on_resident_relations(gpu):
         first = [srv.wait(i) for i in ids]
         assert all("VT" not in r for r in first)
         for i, (cells, V, s) in zip(ids, reqs):
+            TT_ref, VT_ref = O.tt(cells, V), O.vt(cells, V)
+            # built and kept on the device: counts only
+            r0 = srv.followup(i, VT | TT, copy=False)
+            assert r0["tt_nnz"] == len(TT_ref.val) and r0["vt_nnz"] == len(VT_ref.val)
+            assert r0["copy_ms"] == 0 and r0["followup_ms"] >= 0
             r = srv.followup(i, VT | TT)
-            assert np.array_equal(r["TT"][1], O.tt(cells, V).val[:, 0])
-            assert np.array_equal(r["VT"][1], O.vt(cells, V).val[:, 0])
+            assert np.array_equal(r["TT"][0], TT_ref.off)
+            assert np.array_equal(r["TT"][1], TT_ref.val[:, 0])
+            assert np.array_equal(r["VT"][1], VT_ref.val[:, 0])
+            assert r["upload_ms"] >= 0 and r["compute_ms"] >= 0
             srv.release(i)
 
 

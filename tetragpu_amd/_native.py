@@ -80,6 +80,9 @@ _HIP_SIGS = {
     "tg_profile_read": (C.c_int, [P, I32, C.POINTER(I32), P, P]),
     "tg_vv_critpoints_noalloc": (C.c_int, [P, I64, I64, P, C.c_int, P, P, I64, P, P, P]),
     "tg_noalloc_limits": (C.c_int, [I64, I64, C.POINTER(I64), C.POINTER(I64)]),
+    "tg_vv_footprint": (C.c_int, [I64, I64, C.POINTER(I64)]),
+    "tg_device_budget": (C.c_int, [C.POINTER(I64)]),
+    "tg_critpoints_split": (C.c_int, [P, I64, I64, P, C.c_int, P, I32, P, I64, P, P]),
 }
 
 _HOST_SIGS = {

@@ -2,7 +2,8 @@
 // options, help texts, defaults and flag-bit semantics.  --validate is
 // always available here; bit 0 starts at the bitset default like every other
 // bit (flags can only be switched on), and --no_validate (an option of this
-// library) switches validation off.
+// library) switches validation off.  --export_vertices (also this library's)
+// adds the per-vertex lines to the class export.
 #include "argparse.h"
 
 #include <cstdio>
@@ -67,6 +68,7 @@ void parse(int argc, char *argv[], runtime_arguments &args) {
         {"build_FF", no_argument, nullptr, 0},
         {"build_EE", no_argument, nullptr, 0},
         {"build_VV", no_argument, nullptr, 0},
+        {"export_vertices", no_argument, nullptr, 0},
         {0, 0, 0, 0}};
     // flag-only options point at their bit (bit 0 validate, 1..13 relations)
     long_opts[11].flag = &bits[0];
@@ -77,6 +79,9 @@ void parse(int argc, char *argv[], runtime_arguments &args) {
         long_opts[12 + b].flag = &bits[b];
         long_opts[12 + b].val = 1;
     }
+    int export_vertices = 0;
+    long_opts[26].flag = &export_vertices;
+    long_opts[26].val = 1;
     const option_map help_info = {
         {"help", "Print this help message and exit"},
         {"input", "Tetrahedral mesh input (.vtu, or kuhn:N[:seed[:scalar]] for a generated grid)"},
@@ -99,6 +104,7 @@ void parse(int argc, char *argv[], runtime_arguments &args) {
         {"build_VT", "Build the VT relationship"}, {"build_TT", "Build the TT relationship"},
         {"build_FF", "Build the FF relationship"}, {"build_EE", "Build the EE relationship"},
         {"build_VV", "Build the VV relationship"},
+        {"export_vertices", "Also export one 'A <vertex> <class>' line per classified vertex"},
     };
     const option_map metavars = {
         {"input", "input.vtu"},
@@ -223,6 +229,7 @@ void parse(int argc, char *argv[], runtime_arguments &args) {
     for (int b = 0; b < 14; b++)
         if (!args.arg_flags[b]) args.arg_flags[b] = bits[b] != 0;
     if (no_validate) args.arg_flags[0] = false;
+    if (export_vertices) args.export_vertices = true;
     if (args.debug >= DEBUG_MIN)
         for (int b = 0; b < 14; b++)
             std::cout << "Bit flag [" << b << ": " << args.flag_names[b]

@@ -9,6 +9,7 @@
 #include <string>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <fstream>
 #include <iostream>
 #include <memory>
@@ -34,6 +35,11 @@ void export_classes(unsigned int *classes, TV_Data &TV, runtime_arguments &args)
     // class 2, "maxima" <- class 1 (its macro names, critPoints.cu:18-21)
     const bool parts = !args.no_partitioning && TV.partitionIDs;
     const int np = parts ? std::max(TV.n_partitions, 1) : 1;
+    // names by class number, as the reference prints them (critPoints.cu:362)
+    static const char *const names[5] = {"NULL", "max", "min", "regular", "saddle"};
+    // the partition whose non-regular vertices get a line of their own
+    // (critPoints.cu:27, DETAILED_PARTITION_INFORMATION; lines at :402-409)
+    constexpr int kDetailPartition = 1;
     std::vector<std::array<long long, 5>> n_class((size_t)np, std::array<long long, 5>{});
     long long insane = 0;
     for (vtkIdType v = 0; v < TV.nPoints; v++) {
@@ -43,6 +49,11 @@ void export_classes(unsigned int *classes, TV_Data &TV, runtime_arguments &args)
         if ((a >= 1 && b == 0 && c != 2) || (a == 0 && b >= 1 && c != 1) ||
             (a == 1 && b == 1 && c != 3) || (a > 1 && b > 1 && c != 4))
             insane++;
+        if (args.export_vertices && c > 0) out << "A " << v << " " << c << "\n";
+        if (p == kDetailPartition && c > 0 && c != 3)
+            out << "Partition " << kDetailPartition << " has a " << (c <= 4 ? names[c] : "?")
+                << " for global vertex ID " << v << " (upper: " << a << ", lower: " << b
+                << ")\n";
         if (c <= 4 && p >= 0 && p < np) n_class[(size_t)p][c]++;
     }
     if (insane)
@@ -68,6 +79,31 @@ void export_classes(unsigned int *classes, TV_Data &TV, runtime_arguments &args)
     }
 }
 
+int64_t device_budget_bytes() {
+    const char *env = std::getenv("TETRA_HBM_BUDGET");
+    if (env && *env) return std::strtoll(env, nullptr, 10);
+    return 0;  // tg_critpoints_split: 90 % of the free device memory
+}
+
+namespace {
+
+// int32 copies of TV's cells (range-checked) for the C ABI.
+std::vector<int32_t> cells_i32(const TV_Data &TV) {
+    std::vector<int32_t> c((size_t)(4 * TV.nCells));
+    for (size_t i = 0; i < c.size(); i++) {
+        const vtkIdType x = TV.cells[i];
+        if (x < 0 || x >= TV.nPoints) {
+            std::fprintf(stderr, "tetra: cell %zu references vertex %lld outside [0, %lld)\n",
+                         i / 4, (long long)x, (long long)TV.nPoints);
+            std::exit(EXIT_FAILURE);
+        }
+        c[i] = (int32_t)x;
+    }
+    return c;
+}
+
+}  // namespace
+
 unsigned int *gpu_critical_points(const TV_Data &TV, const runtime_arguments &args) {
     if (!TV.vertexAttributes) {
         std::fprintf(stderr, "tetra: critical points need a scalar point array\n");
@@ -75,6 +111,35 @@ unsigned int *gpu_critical_points(const TV_Data &TV, const runtime_arguments &ar
     }
     if (!args.no_partitioning && TV.partitionIDs)
         return partitioned_critical_points(TV, args, true);
+    // a mesh whose hot-path footprint exceeds the device budget (or one
+    // handle's 32-bit limits) runs in parts
+    int64_t need = 0, budget = device_budget_bytes();
+    if (budget <= 0 && (tg_init(0) != TG_OK || tg_device_budget(&budget) != TG_OK)) {
+        std::fprintf(stderr, "tetra: device memory query failed: %s\n", tg_last_error());
+        std::exit(EXIT_FAILURE);
+    }
+    tg_vv_footprint(TV.nCells, TV.nPoints, &need);
+    const bool one_handle = 4 * TV.nCells < (int64_t)INT32_MAX / 3;
+    if (need > budget || !one_handle) {
+        const std::vector<int32_t> cells = cells_i32(TV);
+        unsigned int *out = new unsigned int[3 * (size_t)std::max<vtkIdType>(TV.nPoints, 1)];
+        tg_split_stats st{};
+        const int rc = tg_critpoints_split(cells.data(), TV.nCells, TV.nPoints,
+                                           TV.vertexAttributes, 1, nullptr, 0, nullptr, budget,
+                                           out, &st);
+        if (rc != TG_OK) {
+            std::fprintf(stderr, "tetra: critical points in parts failed [%d]: %s\n", rc,
+                         tg_last_error());
+            std::exit(EXIT_FAILURE);
+        }
+        if (args.debug > NO_DEBUG)
+            std::cout << "Mesh footprint " << need << " bytes exceeds the device budget of "
+                      << budget << " bytes: classified in " << st.parts
+                      << " parts (largest " << st.max_cells << " cells, " << st.cells_total
+                      << " cells in all; localisation " << st.localize_s << " s, classification "
+                      << st.classify_s << " s)\n";
+        return out;
+    }
     make_TV_for_GPU(TV);
     // the mesh made by make_TV_for_GPU is the thread's current handle; the
     // C++ layer exposes it through a VV build, then classes come from the
@@ -259,9 +324,10 @@ int localize_partition(const TV_Data &TV, const std::vector<int64_t> &cells,
 
 namespace {
 
-// The loop itself; failures throw PartitionError.
+// The loop on the host (the validation path): localisation and the host
+// builders per partition; failures throw PartitionError.
 std::unique_ptr<unsigned int[]> partition_loop(const TV_Data &TV, const runtime_arguments &args,
-                                               bool on_gpu, PartitionStats *stats) {
+                                               PartitionStats *stats) {
     using clock = std::chrono::steady_clock;
     auto secs = [](clock::time_point a, clock::time_point b) {
         return std::chrono::duration<double>(b - a).count();
@@ -279,10 +345,6 @@ std::unique_ptr<unsigned int[]> partition_loop(const TV_Data &TV, const runtime_
     for (int p = 0; p < np; p++)
         order.emplace_back(TV.n_per_partition ? (int64_t)TV.n_per_partition[p] : 0, p);
     std::sort(order.rbegin(), order.rend());
-    if (on_gpu) {
-        const int rc = tg_init(0);
-        if (rc != TG_OK) part_fail(std::string("tg_init failed: ") + tg_last_error(), rc);
-    }
 
     std::unique_ptr<unsigned int[]> out(
         new unsigned int[3 * (size_t)std::max<vtkIdType>(TV.nPoints, 1)]());
@@ -306,18 +368,7 @@ std::unique_ptr<unsigned int[]> partition_loop(const TV_Data &TV, const runtime_
             std::cout << "works on partition " << p << " with " << nv << " points and " << nc
                       << " cells (" << e.first << " points are within partition)\n";
         local.assign(3 * (size_t)nv, 0u);
-        if (on_gpu) {
-            tg_mesh *m = nullptr;
-            int rc = tg_mesh_create(&m, lm.cells.data(), nc, nv, lm.scalars.data(), 1, 0, nullptr);
-            if (rc == TG_OK) rc = tg_vv_critpoints(m);
-            if (rc == TG_OK) rc = tg_classes_copy(m, local.data());
-            const std::string err = rc == TG_OK ? "" : tg_last_error();
-            if (m) tg_mesh_destroy(m);
-            if (rc != TG_OK)
-                part_fail("critical points of partition " + std::to_string(p) + " failed [" +
-                              std::to_string(rc) + "]: " + err,
-                          rc);
-        } else {
+        {
             TV_Data sub(nv, nc);
             for (size_t i = 0; i < lm.cells.size(); i++) sub.cells[i] = lm.cells[i];
             auto VV = elective_make_VV(sub, nv);
@@ -344,12 +395,48 @@ std::unique_ptr<unsigned int[]> partition_loop(const TV_Data &TV, const runtime_
     return out;
 }
 
+// The loop on the device: localisation and classification per partition in
+// libtetra_hip (tg_critpoints_split with the partition ids), largest first.
+std::unique_ptr<unsigned int[]> partition_loop_device(const TV_Data &TV,
+                                                      const runtime_arguments &args,
+                                                      PartitionStats *stats) {
+    if (!TV.vertexAttributes) part_fail("critical points need a scalar point array");
+    if (!TV.partitionIDs) part_fail("partitioned critical points need partition ids");
+    const int np = std::max(TV.n_partitions, 1);
+    const std::vector<int32_t> cells = cells_i32(TV);
+    std::unique_ptr<unsigned int[]> out(
+        new unsigned int[3 * (size_t)std::max<vtkIdType>(TV.nPoints, 1)]());
+    tg_split_stats st{};
+    const int rc = tg_critpoints_split(cells.data(), TV.nCells, TV.nPoints, TV.vertexAttributes,
+                                       1, TV.partitionIDs, np, TV.partitionCells,
+                                       device_budget_bytes(), out.get(), &st);
+    if (rc != TG_OK)
+        part_fail(std::string("partitioned critical points failed: ") + tg_last_error(),
+                  rc == TG_ERR_ARG ? -1 : rc);
+    if (args.debug > NO_DEBUG)
+        std::cout << "classified " << st.parts << " partitions on the device (largest "
+                  << st.max_cells << " cells; localisation " << st.localize_s
+                  << " s, classification " << st.classify_s << " s)\n";
+    if (stats) {
+        stats->partitions_run = st.parts;
+        stats->max_cells = st.max_cells;
+        stats->max_points = st.max_points;
+        stats->cells_total = st.cells_total;
+        stats->total_s = st.total_s;
+        stats->localize_s = st.localize_s;
+        stats->classify_s = st.classify_s;
+    }
+    return out;
+}
+
 }  // namespace
 
 unsigned int *partitioned_critical_points(const TV_Data &TV, const runtime_arguments &args,
                                           bool on_gpu, PartitionStats *stats) {
     try {
-        return partition_loop(TV, args, on_gpu, stats).release();
+        return (on_gpu ? partition_loop_device(TV, args, stats)
+                       : partition_loop(TV, args, stats))
+            .release();
     } catch (const PartitionError &e) {
         std::fprintf(stderr, "tetra: %s\n", e.msg.c_str());
         std::exit(EXIT_FAILURE);
@@ -390,7 +477,7 @@ extern "C" int tetra_partitioned_classes(const int32_t *cells, int64_t n_cells, 
     args.debug = NO_DEBUG;
     PartitionStats st;
     try {
-        std::unique_ptr<unsigned int[]> out = partition_loop(tv, args, true, &st);
+        std::unique_ptr<unsigned int[]> out = partition_loop_device(tv, args, &st);
         std::copy(out.get(), out.get() + 3 * n_points, classes);
     } catch (const PartitionError &e) {
         g_partition_error = e.msg;

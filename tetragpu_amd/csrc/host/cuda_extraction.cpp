@@ -5,6 +5,8 @@
 // it has not seen), so worker threads do not share state.
 #include "cuda_extraction.h"
 
+#include <atomic>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -297,6 +299,34 @@ tg_mesh *tetra_current_mesh() { return g_session.mesh; }
 
 void make_TV_for_GPU(const TV_Data &tv_relationship) { upload(tv_relationship); }
 
+// Transfer and translation intervals of the host-returning builders, in the
+// reference timer's line format (src/metrics.cpp:104-113: seconds, then
+// microseconds zero-padded to six digits).
+namespace {
+std::atomic<bool> g_report_times{false};
+thread_local std::vector<HostTiming> g_times;
+constexpr const char *kTransfer = "GPU Device->Host transfer";
+constexpr const char *kTranslation = "GPU Device->Host translation";
+
+struct Stopwatch {
+    const char *timer;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    explicit Stopwatch(const char *name) : timer(name) {}
+    void lap(const char *label) {
+        const auto t1 = std::chrono::steady_clock::now();
+        const double s = std::chrono::duration<double>(t1 - t0).count();
+        t0 = t1;
+        g_times.push_back(HostTiming{timer, label, s});
+        if (g_report_times.load(std::memory_order_relaxed)) {
+            const long long us = (long long)(s * 1e6);
+            std::printf("\u23f3Timer[%s] %s: %lld.%06lld\n", timer, label, us / 1000000,
+                        us % 1000000);
+            std::fflush(stdout);
+        }
+    }
+};
+}  // namespace
+
 // The reference declares (table, n_edges, n_verts) (include/cuda_extraction.h:
 // 15-17) but defines and calls it as (table, n_verts, n_edges)
 // (src/cuda_extraction.cu:85-88, :271); the count equal to the table's row
@@ -332,12 +362,15 @@ std::unique_ptr<EV_Data> make_EV_GPU(const VE_Data &VE, const vtkIdType, const v
     tg_mesh *m = current("make_EV_GPU");
     expect_count("edges", n_edges(m), n_edges_);
     const IdMap &em = edge_map(m, VE);
+    Stopwatch sw("EV_GPU");
     HostRel h = fetch(m, TG_REL_EV);
+    sw.lap(kTransfer);
     if (!em.identity) {
         const auto v = renumbered(h, &em, nullptr);
         h.val.assign(v.begin(), v.end());
     }
     auto out = std::make_unique<EV_Data>(to_fixed<2>(h));
+    sw.lap(kTranslation);
     finish(free_transients);
     return out;
 }
@@ -362,12 +395,15 @@ std::unique_ptr<TF_Data> make_TF_GPU(const TV_Data &TV, const VF_Data &VF, const
     expect_count("faces", n_faces(m), n_faces_);
     const IdMap &fm = face_map(m, VF);
     build(m, TG_REL_TF);
+    Stopwatch sw("TF_GPU");
     HostRel h = fetch(m, TG_REL_TF);
+    sw.lap(kTransfer);
     if (!fm.identity) {
         const auto v = renumbered(h, nullptr, &fm);
         h.val.assign(v.begin(), v.end());
     }
     auto out = std::make_unique<TF_Data>(to_fixed<4>(h));
+    sw.lap(kTranslation);
     finish(free_transients);
     return out;
 }
@@ -392,12 +428,15 @@ std::unique_ptr<TE_Data> make_TE_GPU(const TV_Data &TV, const VE_Data &VE, const
     expect_count("edges", n_edges(m), n_edges_);
     const IdMap &em = edge_map(m, VE);
     build(m, TG_REL_TE);
+    Stopwatch sw("TE_GPU");
     HostRel h = fetch(m, TG_REL_TE);
+    sw.lap(kTransfer);
     if (!em.identity) {
         const auto v = renumbered(h, nullptr, &em);
         h.val.assign(v.begin(), v.end());
     }
     auto out = std::make_unique<TE_Data>(to_fixed<6>(h));
+    sw.lap(kTranslation);
     finish(free_transients);
     return out;
 }
@@ -418,7 +457,9 @@ std::unique_ptr<FV_Data> make_FV_GPU(const VF_Data &VF, const vtkIdType, const v
     tg_mesh *m = current("make_FV_GPU");
     expect_count("faces", n_faces(m), n_faces_);
     const IdMap &fm = face_map(m, VF);
+    Stopwatch sw("FV_GPU");
     HostRel h = fetch(m, TG_REL_FV);
+    sw.lap(kTransfer);
     if (!fm.identity) {
         const auto v = renumbered(h, &fm, nullptr);
         h.val.assign(v.begin(), v.end());
@@ -426,6 +467,7 @@ std::unique_ptr<FV_Data> make_FV_GPU(const VF_Data &VF, const vtkIdType, const v
     auto out = std::make_unique<FV_Data>((size_t)h.rows);
     for (int64_t f = 0; f < h.rows; f++)
         (*out)[f] = FaceData(h.val[3 * f], h.val[3 * f + 1], h.val[3 * f + 2]);
+    sw.lap(kTranslation);
     finish(free_transients);
     return out;
 }
@@ -455,7 +497,9 @@ std::unique_ptr<FE_Data> make_FE_GPU(const VF_Data &VF, const VE_Data &VE, const
     const IdMap &em = edge_map(m, VE);
     const IdMap &fm = face_map(m, VF);
     build(m, TG_REL_FE);
+    Stopwatch sw("FE_GPU");
     HostRel h = fetch(m, TG_REL_FE);
+    sw.lap(kTransfer);
     if (!em.identity || !fm.identity) {
         const auto v = renumbered(h, &fm, &em, 2, 1);
         h.val.assign(v.begin(), v.end());
@@ -464,6 +508,7 @@ std::unique_ptr<FE_Data> make_FE_GPU(const VF_Data &VF, const VE_Data &VE, const
     for (int64_t f = 0; f < h.rows; f++)
         for (int k = 0; k < 3; k++)
             (*out)[f][k] = EdgeData(h.val[6 * f + 2 * k], h.val[6 * f + 2 * k + 1]);
+    sw.lap(kTranslation);
     finish(free_transients);
     return out;
 }
@@ -491,7 +536,11 @@ std::unique_ptr<VV_Data> make_VV_GPU(const TV_Data &TV, const int, const int,
                                      const bool free_transients) {
     tg_mesh *m = mesh_for(TV);
     build(m, TG_REL_VV);
-    auto out = std::make_unique<VV_Data>(to_rows(fetch(m, TG_REL_VV)));
+    Stopwatch sw("VV_GPU");
+    const HostRel h = fetch(m, TG_REL_VV);
+    sw.lap(kTransfer);
+    auto out = std::make_unique<VV_Data>(to_rows(h));
+    sw.lap(kTranslation);
     finish(free_transients);
     return out;
 }
@@ -523,38 +572,55 @@ device_VT *make_VT_GPU_return(const VT_Data &VT) {
 namespace {
 template <int REL>
 std::unique_ptr<std::vector<std::vector<vtkIdType>>> csr_relation(const TV_Data &TV,
-                                                                   bool free_transients) {
+                                                                   bool free_transients,
+                                                                   const char *timer) {
     tg_mesh *m = mesh_for(TV);
     build(m, REL);
-    auto out = std::make_unique<std::vector<std::vector<vtkIdType>>>(to_rows(fetch(m, REL)));
+    Stopwatch sw(timer);
+    const HostRel h = fetch(m, REL);
+    sw.lap(kTransfer);
+    auto out = std::make_unique<std::vector<std::vector<vtkIdType>>>(to_rows(h));
+    sw.lap(kTranslation);
     finish(free_transients);
     return out;
 }
 }  // namespace
 
 std::unique_ptr<ET_Data> make_ET_GPU(const TV_Data &TV, const bool f) {
-    return csr_relation<TG_REL_ET>(TV, f);
+    return csr_relation<TG_REL_ET>(TV, f, "ET_GPU");
 }
 std::unique_ptr<EF_Data> make_EF_GPU(const TV_Data &TV, const bool f) {
-    return csr_relation<TG_REL_EF>(TV, f);
+    return csr_relation<TG_REL_EF>(TV, f, "EF_GPU");
 }
 std::unique_ptr<TT_Data> make_TT_GPU(const TV_Data &TV, const bool f) {
-    return csr_relation<TG_REL_TT>(TV, f);
+    return csr_relation<TG_REL_TT>(TV, f, "TT_GPU");
 }
 std::unique_ptr<FF_Data> make_FF_GPU(const TV_Data &TV, const bool f) {
-    return csr_relation<TG_REL_FF>(TV, f);
+    return csr_relation<TG_REL_FF>(TV, f, "FF_GPU");
 }
 std::unique_ptr<EE_Data> make_EE_GPU(const TV_Data &TV, const bool f) {
-    return csr_relation<TG_REL_EE>(TV, f);
+    return csr_relation<TG_REL_EE>(TV, f, "EE_GPU");
 }
 std::unique_ptr<VT_Data> make_VT_GPU(const TV_Data &TV, const bool f) {
-    return csr_relation<TG_REL_VT>(TV, f);
+    return csr_relation<TG_REL_VT>(TV, f, "VT_GPU");
 }
 
 std::unique_ptr<FT_Data> make_FT_GPU(const TV_Data &TV, const bool free_transients) {
     tg_mesh *m = mesh_for(TV);
     build(m, TG_REL_FT);
-    auto out = std::make_unique<FT_Data>(to_fixed<2>(fetch(m, TG_REL_FT)));
+    Stopwatch sw("FT_GPU");
+    const HostRel h = fetch(m, TG_REL_FT);
+    sw.lap(kTransfer);
+    auto out = std::make_unique<FT_Data>(to_fixed<2>(h));
+    sw.lap(kTranslation);
     finish(free_transients);
+    return out;
+}
+
+void set_host_timing_report(bool on) { g_report_times.store(on, std::memory_order_relaxed); }
+
+std::vector<HostTiming> take_host_timings() {
+    std::vector<HostTiming> out;
+    out.swap(g_times);
     return out;
 }

@@ -73,6 +73,7 @@ struct Request {
     uint32_t followups = 0;
     int64_t bytes = 0;
     Clock::time_point submitted, started, finished;
+    double upload_ms = 0, compute_ms = 0, followup_ms = 0, copy_ms = 0;
     bool done = false;
     int status = TG_OK;
     std::shared_ptr<Group> group;
@@ -81,6 +82,7 @@ struct Request {
     Pinned<uint32_t> classes;
     Pinned<int64_t> vt_off, tt_off;
     Pinned<int32_t> vt_val, tt_val;
+    int64_t vt_nnz = 0, tt_nnz = 0;
     bool has_vt = false, has_tt = false;
 };
 
@@ -100,6 +102,8 @@ struct Group {
     }
 };
 
+constexpr uint32_t kRelFlags = TG_FLAG(TG_REL_VT) | TG_FLAG(TG_REL_TT);
+
 int copy_csr(tg_mesh *m, int rel, Pinned<int64_t> &off, Pinned<int32_t> &val) {
     int64_t rows = 0, nnz = 0;
     int32_t width = 0, fixed = 0;
@@ -118,25 +122,45 @@ int copy_batch_csr(tg_batch *b, int32_t mesh, int rel, Pinned<int64_t> &off,
     return tg_batch_relation_copy(b, mesh, rel, off.data(), val.data(), &nnz);
 }
 
-// VT / TT of one request on its group's resident state (group mutex held).
+// Entries of a resident relation for one request.
+int resident_nnz(Group &g, const Request &r, int rel, int64_t *nnz) {
+    if (g.batch) return tg_batch_relation_copy(g.batch, r.index, rel, nullptr, nullptr, nnz);
+    int64_t rows = 0;
+    int32_t width = 0, fixed = 0;
+    return tg_relation_shape(g.mesh, rel, &rows, nnz, &width, &fixed);
+}
+
+// VT / TT of one request on its group's resident state (group mutex held):
+// built on the device once per group; copied to the request's host arrays
+// only with TG_SERVE_COPY.
 int followup_locked(Group &g, Request &r, uint32_t flags) {
-    const uint32_t need = flags & ~g.built;
+    const auto t0 = Clock::now();
+    const uint32_t rels = flags & kRelFlags;
+    const uint32_t need = rels & ~g.built;
     int rc = TG_OK;
     if (need) rc = g.batch ? tg_batch_build(g.batch, need) : tg_build(g.mesh, need);
     if (rc) return rc;
     g.built |= need;
-    if ((flags & TG_FLAG(TG_REL_VT)) && !r.has_vt) {
+    // counts (they synchronise the group's stream, so the build time is whole)
+    if ((rels & TG_FLAG(TG_REL_VT)) && (rc = resident_nnz(g, r, TG_REL_VT, &r.vt_nnz))) return rc;
+    if ((rels & TG_FLAG(TG_REL_TT)) && (rc = resident_nnz(g, r, TG_REL_TT, &r.tt_nnz))) return rc;
+    const auto t1 = Clock::now();
+    r.followup_ms = ms_between(t0, t1);
+    r.copy_ms = 0;
+    if (!(flags & TG_SERVE_COPY)) return TG_OK;
+    if ((rels & TG_FLAG(TG_REL_VT)) && !r.has_vt) {
         rc = g.batch ? copy_batch_csr(g.batch, r.index, TG_REL_VT, r.vt_off, r.vt_val, r.V)
                      : copy_csr(g.mesh, TG_REL_VT, r.vt_off, r.vt_val);
         if (rc) return rc;
         r.has_vt = true;
     }
-    if ((flags & TG_FLAG(TG_REL_TT)) && !r.has_tt) {
+    if ((rels & TG_FLAG(TG_REL_TT)) && !r.has_tt) {
         rc = g.batch ? copy_batch_csr(g.batch, r.index, TG_REL_TT, r.tt_off, r.tt_val, r.T)
                      : copy_csr(g.mesh, TG_REL_TT, r.tt_off, r.tt_val);
         if (rc) return rc;
         r.has_tt = true;
     }
+    r.copy_ms = ms_between(t1, Clock::now());
     return TG_OK;
 }
 
@@ -147,14 +171,18 @@ void fill_result(const Request &r, tg_request_result *o) {
     o->classes = r.classes.data();
     o->vt_off = r.has_vt ? r.vt_off.data() : nullptr;
     o->vt_val = r.has_vt ? r.vt_val.data() : nullptr;
-    o->vt_nnz = r.has_vt ? r.vt_off.back() : 0;
+    o->vt_nnz = r.vt_nnz;
     o->tt_off = r.has_tt ? r.tt_off.data() : nullptr;
     o->tt_val = r.has_tt ? r.tt_val.data() : nullptr;
-    o->tt_nnz = r.has_tt ? r.tt_off.back() : 0;
+    o->tt_nnz = r.tt_nnz;
     o->queue_ms = ms_between(r.submitted, r.started);
     o->run_ms = ms_between(r.started, r.finished);
     o->total_ms = ms_between(r.submitted, r.finished);
     o->group_size = r.group_size;
+    o->upload_ms = r.upload_ms;
+    o->compute_ms = r.compute_ms;
+    o->followup_ms = r.followup_ms;
+    o->copy_ms = r.copy_ms;
 }
 
 }  // namespace
@@ -204,9 +232,11 @@ int run_group(std::vector<Request *> &reqs, Group &g) {
     const auto t0 = Clock::now();
     for (Request *r : reqs) r->started = t0;
     int rc = TG_OK;
+    Clock::time_point t1 = t0;
     if (reqs.size() == 1) {
         Request &r = *reqs[0];
         rc = tg_mesh_create(&g.mesh, r.cells, r.T, r.V, r.scalars, 0, 0, g.stream);
+        t1 = Clock::now();
         if (!rc) rc = tg_vv_critpoints(g.mesh);
         if (!rc && !r.classes.resize(3 * (size_t)std::max<int64_t>(r.V, 1))) rc = TG_ERR_ALLOC;
         if (!rc) rc = tg_classes_copy(g.mesh, r.classes.data());
@@ -225,6 +255,7 @@ int run_group(std::vector<Request *> &reqs, Group &g) {
         }
         rc = tg_batch_create(&g.batch, n, coff.data(), cells.data(), poff.data(), sc.data(), 0, 0,
                              g.stream);
+        t1 = Clock::now();
         if (!rc) rc = tg_batch_vv_critpoints(g.batch);
         std::vector<uint32_t> all(3 * (size_t)std::max<int64_t>(poff[n], 1));
         if (!rc) rc = tg_batch_classes_copy(g.batch, all.data());
@@ -239,7 +270,12 @@ int run_group(std::vector<Request *> &reqs, Group &g) {
                       reqs[i]->classes.data());
         }
     }
-    if (!rc && reqs[0]->followups) {
+    const auto t2 = Clock::now();
+    for (Request *r : reqs) {
+        r->upload_ms = ms_between(t0, t1);
+        r->compute_ms = ms_between(t1, t2);
+    }
+    if (!rc && (reqs[0]->followups & kRelFlags)) {
         std::lock_guard<std::mutex> lk(g.mu);
         for (Request *r : reqs)
             if (!rc) rc = followup_locked(g, *r, r->followups);
@@ -308,7 +344,7 @@ int tg_server_submit(tg_server *s, const int32_t *cells, int64_t n_cells, int64_
                      const float *scalars, uint32_t followups, uint64_t *id) {
     if (!s || !id || n_cells < 0 || n_points < 0 || (n_cells && !cells) || (n_points && !scalars))
         return TG_ERR_ARG;
-    if (followups & ~(TG_FLAG(TG_REL_VT) | TG_FLAG(TG_REL_TT))) return TG_ERR_ARG;
+    if (followups & ~(kRelFlags | TG_SERVE_COPY)) return TG_ERR_ARG;
     auto r = std::make_unique<Request>();
     r->cells = cells;
     r->scalars = scalars;
@@ -342,7 +378,7 @@ int tg_server_wait(tg_server *s, uint64_t id, tg_request_result *out) {
 }
 
 int tg_server_followup(tg_server *s, uint64_t id, uint32_t flags, tg_request_result *out) {
-    if (!s || (flags & ~(TG_FLAG(TG_REL_VT) | TG_FLAG(TG_REL_TT)))) return TG_ERR_ARG;
+    if (!s || (flags & ~(kRelFlags | TG_SERVE_COPY))) return TG_ERR_ARG;
     Request *r = nullptr;
     std::shared_ptr<Group> g;
     {

@@ -40,6 +40,8 @@ void say(const runtime_arguments &args, const char *what, double s) {
 int main(int argc, char *argv[]) {
     runtime_arguments args;
     parse(argc, argv, args);
+    // the builders' device->host transfer and translation intervals
+    set_host_timing_report(args.debug >= DEBUG_MIN);
     Clock clk;
     std::shared_ptr<TV_Data> TV = get_TV_from_VTK(args);
     say(args, "load TV", clk.lap());
@@ -69,6 +71,7 @@ int main(int argc, char *argv[]) {
     make_TV_for_GPU(*TV);
     if (need_edges) make_VE_for_GPU(VE, nE, nP);
     if (need_faces) make_VF_for_GPU(VF, nP, nF);
+    say(args, "TV upload [GPU]", clk.lap());
     const bool check = args.validate();
 
     if (args.build_TE()) {

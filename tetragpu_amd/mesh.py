@@ -62,6 +62,59 @@ def partitioned_classes(cells, n_points: int, scalars, parts):
     return out, stats
 
 
+class _SplitStats(C.Structure):
+    _fields_ = [("parts", C.c_int64), ("max_cells", C.c_int64), ("max_points", C.c_int64),
+                ("cells_total", C.c_int64), ("footprint_bytes", C.c_int64),
+                ("budget_bytes", C.c_int64), ("total_s", C.c_double),
+                ("localize_s", C.c_double), ("classify_s", C.c_double)]
+
+
+def vv_footprint(n_cells: int, n_points: int) -> int:
+    """Device bytes the VV + critical-point path allocates for a mesh of this
+    size (tg_vv_footprint)."""
+    b = C.c_int64(0)
+    _native.check(_native.hip().tg_vv_footprint(int(n_cells), int(n_points), C.byref(b)))
+    return int(b.value)
+
+
+def split_classes(cells, n_points: int, scalars, parts=None, part_cells=None,
+                  budget_bytes: int = 0):
+    """classes (V, 3) computed in parts, each localised on the device
+    (tg_critpoints_split).  parts=None: the whole mesh when its footprint
+    fits budget_bytes (0: 90 % of free device memory), else vertex-id ranges
+    with their neighbours' cells, identical to the whole mesh.  parts (one id
+    per point): the reference's partition loop, optionally with explicit cell
+    rows part_cells (n_parts, T).  Returns (classes, stats)."""
+    cells = np.ascontiguousarray(cells, dtype=np.int32).reshape(-1, 4)
+    sc = np.ascontiguousarray(scalars)
+    if sc.dtype not in (np.float32, np.float64):
+        sc = sc.astype(np.float64)
+    if sc.shape[0] != n_points:
+        raise ValueError("scalars need one entry per point")
+    pa_ptr, n_parts, rows_ptr = None, 0, None
+    if parts is not None:
+        pa = np.ascontiguousarray(parts, dtype=np.int32)
+        if pa.shape[0] != n_points:
+            raise ValueError("parts need one entry per point")
+        n_parts = int(pa.max()) + 1 if n_points else 1
+        pa_ptr = pa.ctypes.data
+        if part_cells is not None:
+            rows = np.ascontiguousarray(part_cells, dtype=np.int32)
+            if rows.shape != (n_parts, cells.shape[0]):
+                raise ValueError("part_cells must be (n_parts, n_cells)")
+            rows_ptr = rows.ctypes.data
+    elif part_cells is not None:
+        raise ValueError("part_cells needs parts")
+    out = np.zeros((n_points, 3), np.uint32)
+    st = _SplitStats()
+    _native.check(_native.hip().tg_critpoints_split(
+        cells.ctypes.data, cells.shape[0], int(n_points), sc.ctypes.data,
+        1 if sc.dtype == np.float64 else 0, pa_ptr, n_parts, rows_ptr, int(budget_bytes),
+        out.ctypes.data, C.byref(st)))
+    stats = {k: getattr(st, k) for k, _ in _SplitStats._fields_}
+    return out, stats
+
+
 def device_count() -> int:
     n = C.c_int(0)
     _native.hip().tg_device_count(C.byref(n))

@@ -10,6 +10,7 @@ from .mesh import REL
 
 VT = 1 << REL["VT"]
 TT = 1 << REL["TT"]
+COPY = 1 << 31  # TG_SERVE_COPY: follow-up relations also copied to host memory
 
 
 class Config(C.Structure):
@@ -25,7 +26,8 @@ class Result(C.Structure):
                 ("tt_off", C.POINTER(C.c_int64)), ("tt_val", C.POINTER(C.c_int32)),
                 ("tt_nnz", C.c_int64),
                 ("queue_ms", C.c_double), ("run_ms", C.c_double), ("total_ms", C.c_double),
-                ("group_size", C.c_int32)]
+                ("group_size", C.c_int32), ("upload_ms", C.c_double), ("compute_ms", C.c_double),
+                ("followup_ms", C.c_double), ("copy_ms", C.c_double)]
 
 
 _SIGS = {
@@ -60,7 +62,9 @@ def result_dict(r: Result) -> dict:
     V, T = r.n_points, r.n_cells
     out = {"status": r.status, "classes": _arr(r.classes, 3 * V, np.uint32).reshape(V, 3),
            "queue_ms": r.queue_ms, "run_ms": r.run_ms, "total_ms": r.total_ms,
-           "group_size": r.group_size}
+           "group_size": r.group_size, "upload_ms": r.upload_ms, "compute_ms": r.compute_ms,
+           "followup_ms": r.followup_ms, "copy_ms": r.copy_ms,
+           "vt_nnz": r.vt_nnz, "tt_nnz": r.tt_nnz}
     if r.vt_off:
         out["VT"] = (_arr(r.vt_off, V + 1, np.int64), _arr(r.vt_val, r.vt_nnz, np.int32))
     if r.tt_off:
@@ -78,6 +82,8 @@ class Server:
         _native.check(self._lib.tg_server_create(C.byref(self._h), C.byref(cfg)))
 
     def submit(self, cells, n_points, scalars, followups=0) -> int:
+        """followups: VT | TT to build with the first result (| COPY for host
+        copies in the result)."""
         cells = np.ascontiguousarray(cells, np.int32).reshape(-1, 4)
         sc = np.ascontiguousarray(scalars, np.float32)
         rid = C.c_uint64()
@@ -94,12 +100,15 @@ class Server:
         return result_dict(r)
 
     def followup(self, rid: int, flags: int, copy: bool = True) -> dict:
-        """VT/TT of the request from its resident relations; copy=False returns
-        only the entry counts (the arrays stay in the server until release)."""
+        """VT/TT of the request, built on the device once and kept resident;
+        copy=True also copies them to host memory.  copy=False returns the
+        entry counts and the phase times only."""
         r = Result()
-        _native.check(self._lib.tg_server_followup(self._h, rid, flags, C.byref(r)))
+        _native.check(self._lib.tg_server_followup(self._h, rid, flags | (COPY if copy else 0),
+                                                   C.byref(r)))
         if not copy:
-            return {"vt_nnz": r.vt_nnz, "tt_nnz": r.tt_nnz}
+            return {"vt_nnz": r.vt_nnz, "tt_nnz": r.tt_nnz, "followup_ms": r.followup_ms,
+                    "copy_ms": r.copy_ms}
         return result_dict(r)
 
     def release(self, rid: int) -> None:
