@@ -172,6 +172,17 @@ def case_split(seed: int = 3, n: int = 4) -> None:
         if name in ("soup_lists", "non_flag"):
             got, st = t.split_classes(cells_p, Vp, sp, parts=parts, part_cells=pc)
             assert np.array_equal(got, O.critpoints_partitioned(cells_p, Vp, sp, parts, pc)), name
+    # an id outside [0, V) is rejected before any kernel indexes with it
+    bad = cells.copy()
+    bad[len(bad) // 2, 1] = V
+    for kw in (dict(budget_bytes=split_budget(len(cells), V, 0.6)),
+               dict(parts=np.zeros(V, np.int32))):
+        try:
+            t.split_classes(bad, V, s, **kw)
+        except _native.NativeError as e:
+            assert e.status == -4 and f"cell {len(bad) // 2}" in str(e), e
+        else:
+            raise AssertionError("expected TG_ERR_MESH")
     for kw in (dict(budget_bytes=4096),
                dict(parts=np.zeros(V, np.int32), budget_bytes=split_budget(len(cells), V, 0.2))):
         try:

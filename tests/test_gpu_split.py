@@ -98,3 +98,15 @@ def test_partitioned_classes_runs_on_device(gpu):
     got, st = t.partitioned_classes(m.cells, m.n_points, s, parts)
     assert st["partitions"] == 4
     assert np.array_equal(got, O.critpoints_partitioned(m.cells, m.n_points, s, parts))
+
+
+def test_ids_outside_the_mesh_rejected(gpu):
+    m = t.kuhn(8, seed=4, permute_vertices=False, permute_cells=False)
+    bad = m.cells.copy()
+    bad[100, 2] = m.n_points + 5
+    for kw in (dict(budget_bytes=resident(m.n_cells, m.n_points) +
+                    t.vv_footprint(m.n_cells, m.n_points) // 3),
+               dict(parts=np.zeros(m.n_points, np.int32))):
+        with pytest.raises(t.mesh._native.NativeError) as e:
+            t.split_classes(bad, m.n_points, m.scalars, **kw)
+        assert e.value.status == -4 and "cell 100" in str(e.value)

@@ -133,6 +133,18 @@ __global__ void k_scatter_owned(int64_t nv, const int32_t *__restrict__ l2g, Own
     }
 }
 
+// *bad = 1 + the first cell (lowest index seen) holding an id outside [0, V).
+__global__ void k_check_ids(const int4 *__restrict__ tv, int64_t T, int64_t V,
+                            unsigned long long *__restrict__ bad) {
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < T;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        const int4 q = tv[c];
+        const bool out = q.x < 0 || q.y < 0 || q.z < 0 || q.w < 0 || q.x >= V || q.y >= V ||
+                         q.z >= V || q.w >= V;
+        if (out) atomicMin(bad, (unsigned long long)c + 1);
+    }
+}
+
 // used[u] = 1 for every vertex of a cell.
 __global__ void k_used(const int4 *__restrict__ tv, int64_t T, uint8_t *__restrict__ used) {
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < T;
@@ -232,6 +244,17 @@ struct Splitter {
         if (T > 0) TG_HIP(hipMemcpyAsync(tv.p, cells, 16 * (size_t)T, hipMemcpyHostToDevice, s));
         if (V > 0)
             TG_HIP(hipMemcpyAsync(sc.p, scalars, ssize() * (size_t)V, hipMemcpyHostToDevice, s));
+        // every later kernel indexes per-vertex arrays with the cells' ids
+        unsigned long long bad = ~0ull;
+        TG_HIP(hipMemcpyAsync(counter.p, &bad, sizeof bad, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_check_ids, dim3(grid(T)), dim3(SPLIT_BLOCK), 0, s,
+                           tv.as<const int4>(), T, V, counter.as<unsigned long long>());
+        TG_LAUNCH_CHECK("k_check_ids");
+        TG_HIP(hipMemcpyAsync(&bad, counter.p, sizeof bad, hipMemcpyDeviceToHost, s));
+        TG_HIP(hipStreamSynchronize(s));
+        if (bad != ~0ull)
+            return fail_msg(TG_ERR_MESH, "cell " + std::to_string(bad - 1) +
+                                             " holds a vertex id outside [0, n_points)");
         return tg::fill_i32(out.as<int32_t>(), 0, 3 * V, s);
     }
 

@@ -87,19 +87,25 @@ int64_t device_budget_bytes() {
 
 namespace {
 
-// int32 copies of TV's cells (range-checked) for the C ABI.
-std::vector<int32_t> cells_i32(const TV_Data &TV) {
+// int32 copies of TV's cells for the C ABI; *bad = the first entry whose
+// id is outside [0, nPoints), else -1.
+std::vector<int32_t> cells_i32(const TV_Data &TV, int64_t *bad) {
     std::vector<int32_t> c((size_t)(4 * TV.nCells));
+    *bad = -1;
     for (size_t i = 0; i < c.size(); i++) {
         const vtkIdType x = TV.cells[i];
         if (x < 0 || x >= TV.nPoints) {
-            std::fprintf(stderr, "tetra: cell %zu references vertex %lld outside [0, %lld)\n",
-                         i / 4, (long long)x, (long long)TV.nPoints);
-            std::exit(EXIT_FAILURE);
+            *bad = (int64_t)i;
+            break;
         }
         c[i] = (int32_t)x;
     }
     return c;
+}
+
+std::string bad_cell_message(const TV_Data &TV, int64_t i) {
+    return "cell " + std::to_string(i / 4) + " references vertex " +
+           std::to_string((long long)TV.cells[i]) + " outside the mesh";
 }
 
 }  // namespace
@@ -121,7 +127,12 @@ unsigned int *gpu_critical_points(const TV_Data &TV, const runtime_arguments &ar
     tg_vv_footprint(TV.nCells, TV.nPoints, &need);
     const bool one_handle = 4 * TV.nCells < (int64_t)INT32_MAX / 3;
     if (need > budget || !one_handle) {
-        const std::vector<int32_t> cells = cells_i32(TV);
+        int64_t bad = -1;
+        const std::vector<int32_t> cells = cells_i32(TV, &bad);
+        if (bad >= 0) {
+            std::fprintf(stderr, "tetra: %s\n", bad_cell_message(TV, bad).c_str());
+            std::exit(EXIT_FAILURE);
+        }
         unsigned int *out = new unsigned int[3 * (size_t)std::max<vtkIdType>(TV.nPoints, 1)];
         tg_split_stats st{};
         const int rc = tg_critpoints_split(cells.data(), TV.nCells, TV.nPoints,
@@ -403,7 +414,9 @@ std::unique_ptr<unsigned int[]> partition_loop_device(const TV_Data &TV,
     if (!TV.vertexAttributes) part_fail("critical points need a scalar point array");
     if (!TV.partitionIDs) part_fail("partitioned critical points need partition ids");
     const int np = std::max(TV.n_partitions, 1);
-    const std::vector<int32_t> cells = cells_i32(TV);
+    int64_t bad = -1;
+    const std::vector<int32_t> cells = cells_i32(TV, &bad);
+    if (bad >= 0) part_fail(bad_cell_message(TV, bad));
     std::unique_ptr<unsigned int[]> out(
         new unsigned int[3 * (size_t)std::max<vtkIdType>(TV.nPoints, 1)]());
     tg_split_stats st{};
