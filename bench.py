@@ -112,7 +112,15 @@ def summarize(d: Path, workload: str) -> int:
     import csv
 
     prof = ROOT / "profiles"
-    dig = kernel_digest()
+    # the digest of the kernel sources the runs measured, from their JSON lines
+    digs = set()
+    for log_file in d.glob("*.log"):
+        for line in log_file.read_text(errors="replace").splitlines():
+            if line.startswith("{") and '"kernel_digest"' in line:
+                digs.add(json.loads(line)["kernel_digest"])
+    if len(digs) > 1:
+        raise SystemExit(f"runs in {d} measured different kernel sources: {sorted(digs)}")
+    dig = digs.pop() if digs else kernel_digest()
     runs = None
 
     def n_runs(c):
@@ -123,7 +131,8 @@ def summarize(d: Path, workload: str) -> int:
         scale, per = {}, {}
         for c in ("FETCH_SIZE", "WRITE_SIZE"):
             cp = read_counters(d / f"copy_{c}" / "run_counter_collection.csv")
-            vals = [v for k, x in cp.items() for v in x.get(c, [])]
+            # the copies only (the run also fills the source once)
+            vals = [v for k, x in cp.items() if "copyBuffer" in k for v in x.get(c, [])]
             # a 1 GiB copy reads and writes 1 GiB: 1048576 KiB per dispatch
             scale[c] = 1048576.0 / (sum(vals) / len(vals))
             bc = read_counters(d / f"bench_{c}" / "run_counter_collection.csv")
@@ -829,7 +838,7 @@ def main() -> int:
         print(json.dumps({"metric": METRIC, "value": round(T / (ms * 1e-3), 1), "unit": "tets/s",
                           "ms_per_step": round(ms, 4), "hot_path_only": True,
                           "hot_path_runs": max(args.warmup, 1) + 2 * args.steps + 1,
-                          "roofline": roofline}))
+                          "kernel_digest": kernel_digest(), "roofline": roofline}))
         return 0
     # end to end from host buffers: upload TV + scalars, run, copy VV + classes back
     t0 = time.perf_counter()
