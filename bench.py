@@ -367,7 +367,7 @@ def bench_batch(t, torch, n_meshes: int, reps: int, seed: int = 7) -> dict:
             "identical_to_loop": same, "graph": True}
 
 
-def bench_serve(t, rates, n_requests: int, seed: int = 11) -> dict:
+def bench_serve(t, rates, n_requests: int, seed: int = 11, slots: int = 4) -> dict:
     """Poisson arrivals of VV + critical-point requests (Kuhn grids with 10 %
     splits, T log-uniform in [10k, 5M], Gaussian-mixture scalars) followed by
     a VT + TT query on the resident mesh (built on the device, kept there);
@@ -387,13 +387,14 @@ def bench_serve(t, rates, n_requests: int, seed: int = 11) -> dict:
     log(f"serve: pool of {pool_n} meshes generated")
     out = {"pool_cells_min": int(min(m.n_cells for m in pool)),
            "pool_cells_max": int(max(m.n_cells for m in pool)), "requests_per_rate": n_requests,
+           "slots": slots,
            "rates": []}
     for rate in rates:
         lat1, lat2, cells = [], [], []
         phases = {k: [] for k in ("queue_ms", "upload_ms", "compute_ms", "followup_ms",
                                   "client_ms")}
         lock = threading.Lock()
-        with Server(slots=4, hbm_budget_bytes=96 << 30, fuse_max_cells=1 << 19,
+        with Server(slots=slots, hbm_budget_bytes=96 << 30, fuse_max_cells=1 << 19,
                     fuse_max_meshes=64) as srv, ThreadPoolExecutor(64) as ex:
             def one(k, submitted):
                 m = pool[k]
@@ -445,12 +446,20 @@ def bench_serve(t, rates, n_requests: int, seed: int = 11) -> dict:
 
 def bench_relations(t, torch, mesh) -> dict:
     """First-build time of each relation on a fresh device mesh (prerequisites
-    included, e.g. edge ids for EE), and of all of them in one request."""
+    included, e.g. edge ids for EE), and of all of them in one request.
+
+    Each relation is priced against HBM with its compulsory bytes: TV read
+    once (16 bytes per cell) and the relation written once (int32 values; a
+    CSR relation adds int32 offsets, rows + 1).  Intermediate relations a
+    build needs (edge ids for EE, VT for TT) are not counted, so the fraction
+    is an upper bound on how close the build is to the HBM roofline."""
     names = ["VE", "TE", "EV", "ET", "VF", "TF", "FV", "FE", "FT", "EF", "VT", "TT", "FF",
              "EE", "VV"]
-    out = {"per_relation_ms": {}}
+    T = mesh.n_cells
+    out = {"per_relation_ms": {}, "bytes_alg": {}, "hbm_frac": {},
+           "bytes_rule": "16T + 4 x values + 4 x (rows + 1) for CSR relations"}
 
-    def timed(mask):
+    def timed(mask, name=None):
         stream = torch.cuda.Stream()
         with t.Mesh(mesh.cells, mesh.n_points, mesh.scalars, stream=stream.cuda_stream) as d:
             d.sync()
@@ -464,13 +473,18 @@ def bench_relations(t, torch, mesh) -> dict:
             b.record(stream)
             torch.cuda.synchronize()
             d.sync()
-            return a.elapsed_time(b)
+            shape = d.shape(name) if name else None
+            return a.elapsed_time(b), shape
 
     for name in names:
-        out["per_relation_ms"][name] = round(timed(t.flags(name)), 3)
-    ms = timed(t.ALL_FLAGS)
+        ms, sh = timed(t.flags(name), name)
+        B = 16 * T + 4 * sh["nnz"] * sh["width"] + (0 if sh["fixed"] else 4 * (sh["rows"] + 1))
+        out["per_relation_ms"][name] = round(ms, 3)
+        out["bytes_alg"][name] = int(B)
+        out["hbm_frac"][name] = round(B / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)
+    ms, _ = timed(t.ALL_FLAGS)
     out["all_flags_and_critpoints_ms"] = round(ms, 3)
-    log(f"relations: all flags + critical points {ms:.2f} ms")
+    log(f"relations: all flags + critical points {ms:.2f} ms; HBM fractions {out['hbm_frac']}")
     return out
 
 
@@ -684,6 +698,8 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=1000,
                     help="requests per arrival rate in the serving measurement (0: skip)")
     ap.add_argument("--rates", type=float, nargs="+", default=[100, 500, 1000, 2000, 5000])
+    ap.add_argument("--serve-slots", type=int, default=4,
+                    help="server slots (worker threads with their own streams)")
     ap.add_argument("--no-relations", action="store_true",
                     help="skip the per-relation build times on the large mesh")
     ap.add_argument("--no-irregular", action="store_true",
@@ -858,7 +874,8 @@ def main() -> int:
     irregular = None if args.no_irregular else bench_irregular(t, torch, max(args.steps // 2, 1),
                                                                 check=not args.no_cpu)
     batch = bench_batch(t, torch, args.batch_meshes, 3) if args.batch_meshes > 0 else None
-    serve = bench_serve(t, args.rates, args.serve_requests) if args.serve_requests > 0 else None
+    serve = (bench_serve(t, args.rates, args.serve_requests, slots=args.serve_slots)
+             if args.serve_requests > 0 else None)
 
     line = {
         "metric": METRIC,

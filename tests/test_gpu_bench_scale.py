@@ -183,3 +183,32 @@ def test_cfg4_batch_256(gpu):
             off, vals = b.vv(i)
             assert np.array_equal(off, VV.off) and np.array_equal(vals, VV.val[:, 0]), \
                 f"mesh {i} VV"
+
+
+@pytest.mark.timeout(900)
+def test_above_2_24_vertices_vertex_lane_form(gpu):
+    """Kuhn 256^3 (100,663,296 cells, 16,974,593 > 2^24 vertices): the
+    bucketed form packs 24-bit ids, so the automatic choice takes the
+    vertex-lane form here.  VV and classes against the OpenMP oracle."""
+    import os
+    import time
+
+    n = 256
+    m = gpu.kuhn(n, seed=7, scalar="gaussians")
+    assert m.n_points > 1 << 24
+    log(f"kuhn{n} V={m.n_points} T={m.n_cells}")
+    with gpu.Mesh(m.cells, m.n_points, m.scalars) as d:
+        d.vv_critpoints()
+        d.sync()
+        t0 = time.perf_counter()
+        d.vv_critpoints()
+        d.sync()
+        log(f"vertex-lane form: {(time.perf_counter() - t0) * 1e3:.1f} ms per run")
+        off, vals = d.relation("VV")
+        cls = d.classes()
+    E = 7 * n**3 + 9 * n**2 + 3 * n
+    assert off[-1] == 2 * E
+    VV, ref, _ = O.vv_cp_omp(m.cells, m.n_points, m.scalars, min(16, os.cpu_count() or 1))
+    same(off, VV.off, "VV offsets")
+    same(vals[:, 0], VV.val[:, 0], "VV columns")
+    same(cls, ref, "classes")

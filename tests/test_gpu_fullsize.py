@@ -13,8 +13,8 @@ from oracle import binding as O
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("form", [(2, 0), (4, 0), (1, 0)],
-                         ids=["bucketed-auto", "lean-auto", "vertex-lane"])
+@pytest.mark.parametrize("form", [(2, 0), (3, 0), (4, 0), (1, 0)],
+                         ids=["bucketed-auto", "owner-auto", "lean-auto", "vertex-lane"])
 def test_kuhn128_gaussians_identical_to_oracle(gpu, form):
     n = 128
     m = gpu.kuhn(n, seed=1234, scalar="gaussians")
