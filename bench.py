@@ -367,7 +367,7 @@ def bench_batch(t, torch, n_meshes: int, reps: int, seed: int = 7) -> dict:
             "identical_to_loop": same, "graph": True}
 
 
-def bench_serve(t, rates, n_requests: int, seed: int = 11, slots: int = 4) -> dict:
+def bench_serve(t, rates, n_requests: int, seed: int = 11, slots: int = 8) -> dict:
     """Poisson arrivals of VV + critical-point requests (Kuhn grids with 10 %
     splits, T log-uniform in [10k, 5M], Gaussian-mixture scalars) followed by
     a VT + TT query on the resident mesh (built on the device, kept there);
@@ -698,7 +698,7 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=1000,
                     help="requests per arrival rate in the serving measurement (0: skip)")
     ap.add_argument("--rates", type=float, nargs="+", default=[100, 500, 1000, 2000, 5000])
-    ap.add_argument("--serve-slots", type=int, default=4,
+    ap.add_argument("--serve-slots", type=int, default=8,
                     help="server slots (worker threads with their own streams)")
     ap.add_argument("--no-relations", action="store_true",
                     help="skip the per-relation build times on the large mesh")
