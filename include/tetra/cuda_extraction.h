@@ -105,6 +105,6 @@ struct HostTiming {
     double seconds;
 };
 void set_host_timing_report(bool on);
-std::vector<HostTiming> take_host_timings();  // this thread's, since the last call
+std::vector<HostTiming> take_host_timings();  // this thread's, since the last call (latest 1024 at most)
 
 #endif

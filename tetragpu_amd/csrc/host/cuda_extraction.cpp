@@ -316,6 +316,8 @@ struct Stopwatch {
         const auto t1 = std::chrono::steady_clock::now();
         const double s = std::chrono::duration<double>(t1 - t0).count();
         t0 = t1;
+        // a caller that never takes them keeps the latest 1024 intervals
+        if (g_times.size() >= 1024) g_times.erase(g_times.begin(), g_times.begin() + 512);
         g_times.push_back(HostTiming{timer, label, s});
         if (g_report_times.load(std::memory_order_relaxed)) {
             const long long us = (long long)(s * 1e6);
