@@ -63,19 +63,36 @@ def case_all_relations(seed: int = 3, n: int = 4, variant: int = 0) -> None:
 
 def case_vv_cp(seed: int = 5, n: int = 6, variant: int = 2) -> None:
     """VV + classes in one kernel form, against the oracle."""
+    m = t.kuhn(n, seed=seed)
+    same_vv_cp(m.cells, m.n_points, m.scalars, variant)
+
+
+def same_vv_cp(cells, V, scalars, variant: int) -> None:
     from oracle import binding as O
 
-    m = t.kuhn(n, seed=seed)
-    with t.Mesh(m.cells, m.n_points, m.scalars) as d:
+    with t.Mesh(cells, V, scalars) as d:
         d.set_config(variant=variant)
         d.vv_critpoints()
         off, vals = d.relation("VV")
         cls = d.classes()
-    VV = O.vv(m.cells, m.n_points)
+    VV = O.vv(cells, V)
     S = VV.sorted_rows()
     assert np.array_equal(off, S.off) and np.array_equal(vals[:, 0], S.val[:, 0]), "VV"
-    ref = O.critpoints(VV, m.scalars.astype(np.float64), m.n_points)
+    ref = O.critpoints(VV, np.asarray(scalars, np.float64), V)
     assert np.array_equal(cls, ref), "classes"
+
+
+def case_window(seed: int = 1, V: int = 100, T: int = 760, window: int = 16,
+                variant: int = 2) -> None:
+    """A soup whose cells take four vertices from a window of `window`
+    consecutive ids: rows stay short (at most 2 * window - 2 neighbours)
+    while vertices carry many records, so buckets fill up with records of
+    few distinct keys (and, for wide windows, rows go to the heavy kernel)."""
+    rng = np.random.default_rng(seed)
+    w0 = rng.integers(0, V - window + 1, T)
+    cells = np.array([w + rng.choice(window, 4, replace=False) for w in w0], np.int32)
+    scalars = rng.random(V).astype(np.float32)
+    same_vv_cp(cells, V, scalars, variant)
 
 
 def case_hubs(seed: int = 11, hub_splits: int = 40) -> None:
@@ -194,7 +211,7 @@ def case_split(seed: int = 3, n: int = 4) -> None:
 
 
 CASES = {"split": case_split, "noalloc": case_noalloc, "after_hot_path": case_after_hot_path, "all_relations": case_all_relations,
-         "vv_cp": case_vv_cp, "hubs": case_hubs, "soup": case_soup, "routes": case_routes}
+         "vv_cp": case_vv_cp, "window": case_window, "hubs": case_hubs, "soup": case_soup, "routes": case_routes}
 
 
 def main() -> int:

@@ -40,6 +40,9 @@ CASES = [
     ("routes", 4, 1, 7), ("routes", 4, 2, 8),
     ("all_relations", 3, 4, 0), ("all_relations", 3, 4, 1),
     ("hubs", 11, 40), ("soup", 2, 60, 200),
+    # cells drawn from windows of consecutive ids: many records per vertex,
+    # short rows (window 16) or rows for the heavy kernel (window 40)
+    ("window", 1, 100, 760, 16, 2), ("window", 3, 300, 2200, 40, 4),
     # TE, the vertex-lane hot path, then ET/TT/VF on the same mesh
     ("after_hot_path", 8, 5, 1), ("after_hot_path", 8, 5, 2),
     # the no-allocation entry point: two calls, the second smaller
