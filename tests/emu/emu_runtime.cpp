@@ -8,8 +8,11 @@
 
 #include <sys/mman.h>
 
+#include <atomic>
+#include <condition_variable>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #if defined(__has_feature)
@@ -21,7 +24,7 @@
 
 namespace emu {
 
-dim3 g_threadIdx, g_blockIdx, g_blockDim, g_gridDim;
+thread_local dim3 g_threadIdx, g_blockIdx, g_blockDim, g_gridDim;
 
 namespace {
 
@@ -37,16 +40,25 @@ struct Fiber {
     uint64_t slot = 0;
 };
 
-Fiber g_f[MAX_ITEMS];
-ucontext_t g_sched;
-int g_cur = -1, g_n = 0;
-uint64_t g_part[MAX_ITEMS / 64];  // lanes released together by the last wave barrier
-const std::function<void()> *g_body = nullptr;
-const char *g_name = "";
+// Scheduler state is per host thread: a thread runs one workgroup at a time.
+struct Fibers {
+    Fiber f[MAX_ITEMS];
+    ~Fibers() {
+        for (Fiber &x : f)
+            if (x.stack) munmap(x.stack, STACK);
+    }
+};
+thread_local Fibers g_fibers;
+thread_local Fiber *const g_f = g_fibers.f;
+thread_local ucontext_t g_sched;
+thread_local int g_cur = -1, g_n = 0;
+thread_local uint64_t g_part[MAX_ITEMS / 64];  // lanes released together by the last wave barrier
+thread_local const std::function<void()> *g_body = nullptr;
+thread_local const char *g_name = "";
 #ifdef EMU_ASAN
-void *g_fake = nullptr;
-const void *g_main_bottom = nullptr;  // the scheduler's stack, learned on the first switch
-size_t g_main_size = 0;
+thread_local void *g_fake = nullptr;
+thread_local const void *g_main_bottom = nullptr;  // the scheduler's stack, learned on the first switch
+thread_local size_t g_main_size = 0;
 #endif
 
 void die(const char *what) {
@@ -59,8 +71,7 @@ void die(const char *what) {
 // EMU_YIELD=k: every k-th atomic operation hands the CPU to the next work-item
 // of the block, so lock-free code sees other lanes' updates between its own
 // read and write (lost compare-and-swaps, retried probes).
-int g_yield_every = -1;
-uint64_t g_atomics = 0;
+thread_local uint64_t g_atomics = 0;
 
 void to_sched(State st) {
     g_f[g_cur].st = st;
@@ -167,21 +178,103 @@ struct Capture {
 };
 Capture g_cap;
 
-void exec_grid(const char *name, dim3 grid, dim3 block, const std::function<void()> &fn) {
-    if (block.y != 1 || block.z != 1 || block.x == 0 || block.x > (unsigned)MAX_ITEMS)
-        die("unsupported workgroup shape");
+int env_int(const char *name, int dflt) {
+    const char *e = std::getenv(name);
+    return e ? std::atoi(e) : dflt;
+}
+
+// Workgroups of the grid taken from a shared counter (in order, or from the
+// last one down) until none is left.
+void run_blocks(const char *name, dim3 grid, dim3 block, const std::function<void()> &fn,
+                std::atomic<int64_t> &next, bool reverse) {
     g_name = name;
     g_body = &fn;
     g_blockDim = block;
     g_gridDim = grid;
     g_n = (int)block.x;
-    for (unsigned z = 0; z < grid.z; z++)
-        for (unsigned y = 0; y < grid.y; y++)
-            for (unsigned x = 0; x < grid.x; x++) {
-                g_blockIdx = dim3(x, y, z);
-                run_block();
-            }
+    const int64_t total = (int64_t)grid.x * grid.y * grid.z;
+    for (;;) {
+        int64_t k = next.fetch_add(1);
+        if (k >= total) break;
+        if (reverse) k = total - 1 - k;
+        g_blockIdx = dim3((unsigned)(k % grid.x), (unsigned)(k / grid.x % grid.y),
+                          (unsigned)(k / ((int64_t)grid.x * grid.y)));
+        run_block();
+    }
     g_body = nullptr;
+}
+
+// Persistent host threads for EMU_THREADS > 1 (each keeps its fibers).
+struct Workers {
+    std::mutex mu;
+    std::condition_variable cv, done_cv;
+    std::vector<std::thread> threads;
+    uint64_t gen = 0;
+    int busy = 0;
+    bool quit = false;
+    const char *name = nullptr;
+    dim3 grid, block;
+    const std::function<void()> *fn = nullptr;
+    std::atomic<int64_t> next{0};
+    bool reverse = false;
+
+    explicit Workers(int n) {
+        for (int i = 0; i < n; i++) threads.emplace_back([this]() { loop(); });
+    }
+    ~Workers() {
+        {
+            std::lock_guard<std::mutex> g(mu);
+            quit = true;
+        }
+        cv.notify_all();
+        for (auto &t : threads) t.join();
+    }
+    void loop() {
+        uint64_t seen = 0;
+        for (;;) {
+            std::unique_lock<std::mutex> g(mu);
+            cv.wait(g, [&]() { return quit || gen != seen; });
+            if (quit) return;
+            seen = gen;
+            g.unlock();
+            run_blocks(name, grid, block, *fn, next, reverse);
+            g.lock();
+            if (--busy == 0) done_cv.notify_all();
+        }
+    }
+    void run(const char *nm, dim3 gr, dim3 bl, const std::function<void()> &f, bool rev) {
+        std::unique_lock<std::mutex> g(mu);
+        name = nm;
+        grid = gr;
+        block = bl;
+        fn = &f;
+        reverse = rev;
+        next = 0;
+        busy = (int)threads.size();
+        gen++;
+        cv.notify_all();
+        done_cv.wait(g, [&]() { return busy == 0; });
+    }
+};
+
+void exec_grid(const char *name, dim3 grid, dim3 block, const std::function<void()> &fn) {
+    if (block.y != 1 || block.z != 1 || block.x == 0 || block.x > (unsigned)MAX_ITEMS)
+        die("unsupported workgroup shape");
+    static const int threads = env_int("EMU_THREADS", 1);
+    static const bool reverse = [] {
+        const char *e = std::getenv("EMU_ORDER");
+        return e && std::string(e) == "reverse";
+    }();
+    if (threads <= 1) {
+        std::atomic<int64_t> next{0};
+        run_blocks(name, grid, block, fn, next, reverse);
+        return;
+    }
+    // one launch at a time (host threads of the library may launch together)
+    static std::mutex launch_mu;
+    static Workers *w = new Workers(threads);  // process lifetime
+    std::lock_guard<std::mutex> g(launch_mu);
+    w->run(name, grid, block, fn, reverse);
 }
 
 }  // namespace
@@ -189,12 +282,8 @@ void exec_grid(const char *name, dim3 grid, dim3 block, const std::function<void
 int lane() { return g_cur & 63; }
 
 void atomic_point() {
-    if (g_yield_every < 0) {
-        const char *e = std::getenv("EMU_YIELD");
-        g_yield_every = e ? std::atoi(e) : 0;
-    }
-    if (g_yield_every > 0 && g_cur >= 0 && ++g_atomics % (uint64_t)g_yield_every == 0)
-        to_sched(RUN);
+    static const int every = env_int("EMU_YIELD", 0);
+    if (every > 0 && g_cur >= 0 && ++g_atomics % (uint64_t)every == 0) to_sched(RUN);
 }
 
 void syncthreads() { to_sched(WAIT_BLOCK); }

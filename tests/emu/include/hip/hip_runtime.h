@@ -6,10 +6,16 @@
 // requested size (so AddressSanitizer sees every out-of-bounds access), and
 // streams, events and graphs run their work synchronously, in order.
 //
+// Workgroups run one after another, or, with EMU_THREADS=k (k > 1), on k
+// host threads at once, so that workgroups of one launch interleave their
+// global-memory accesses as on the GPU; each thread has its own copy of the
+// LDS arrays (__shared__ is thread_local) and its own fibers.  EMU_ORDER=
+// reverse hands out the workgroups from the last one down.
+//
 // Limits: one-dimensional workgroups of at most 1024 work-items; a wave op
 // runs over the lanes of the wave that reached it while the others wait at a
 // workgroup barrier or have exited (wave ops in both arms of one divergent
-// branch would merge); memory ordering between workgroups is sequential.
+// branch would merge).
 #pragma once
 
 #include <cstddef>
@@ -26,7 +32,7 @@
 #define __global__
 #define __device__
 #define __host__
-#define __shared__ static
+#define __shared__ static thread_local
 #define __forceinline__ inline
 #define __launch_bounds__(...)
 #define __align__(n) __attribute__((aligned(n)))
@@ -51,7 +57,7 @@ inline float4 make_float4(float x, float y, float z, float w) { return float4{x,
 inline float2 make_float2(float x, float y) { return float2{x, y}; }
 
 namespace emu {
-extern dim3 g_threadIdx, g_blockIdx, g_blockDim, g_gridDim;
+extern thread_local dim3 g_threadIdx, g_blockIdx, g_blockDim, g_gridDim;
 void syncthreads();
 int syncthreads_and(int p);
 uint64_t wave_ballot(int p);
@@ -136,24 +142,34 @@ inline typename std::common_type<A, B>::type max(A a, B b) {
     return (C)a > (C)b ? (C)a : (C)b;
 }
 
-// Work-items of one workgroup never run at the same time, so a plain
-// read-modify-write is atomic with respect to the others.
+// Atomics are real read-modify-writes (workgroups may run on several host
+// threads); each is also a point where EMU_YIELD may switch work-items.
+template <typename T, typename F>
+inline T emu_rmw(T *p, F f) {
+    emu::atomic_point();
+    T o = __atomic_load_n(p, __ATOMIC_SEQ_CST);
+    while (!__atomic_compare_exchange_n(p, &o, f(o), false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {
+    }
+    return o;
+}
 template <typename T, typename U>
-inline T atomicAdd(T *p, U v) { emu::atomic_point(); T o = *p; *p = (T)(o + (T)v); return o; }
+inline T atomicAdd(T *p, U v) { return emu_rmw(p, [&](T o) { return (T)(o + (T)v); }); }
 template <typename T, typename U>
-inline T atomicSub(T *p, U v) { emu::atomic_point(); T o = *p; *p = (T)(o - (T)v); return o; }
+inline T atomicSub(T *p, U v) { return emu_rmw(p, [&](T o) { return (T)(o - (T)v); }); }
 template <typename T, typename U>
-inline T atomicOr(T *p, U v) { emu::atomic_point(); T o = *p; *p = (T)(o | (T)v); return o; }
+inline T atomicOr(T *p, U v) { return emu_rmw(p, [&](T o) { return (T)(o | (T)v); }); }
 template <typename T, typename U>
-inline T atomicAnd(T *p, U v) { emu::atomic_point(); T o = *p; *p = (T)(o & (T)v); return o; }
+inline T atomicAnd(T *p, U v) { return emu_rmw(p, [&](T o) { return (T)(o & (T)v); }); }
 template <typename T, typename U>
-inline T atomicMax(T *p, U v) { emu::atomic_point(); T o = *p; if ((T)v > o) *p = (T)v; return o; }
+inline T atomicMax(T *p, U v) { return emu_rmw(p, [&](T o) { return (T)v > o ? (T)v : o; }); }
 template <typename T, typename U>
-inline T atomicMin(T *p, U v) { emu::atomic_point(); T o = *p; if ((T)v < o) *p = (T)v; return o; }
+inline T atomicMin(T *p, U v) { return emu_rmw(p, [&](T o) { return (T)v < o ? (T)v : o; }); }
 template <typename T, typename U>
-inline T atomicExch(T *p, U v) { emu::atomic_point(); T o = *p; *p = (T)v; return o; }
+inline T atomicExch(T *p, U v) { return emu_rmw(p, [&](T) { return (T)v; }); }
 template <typename T, typename U, typename W>
-inline T atomicCAS(T *p, U cmp, W v) { emu::atomic_point(); T o = *p; if (o == (T)cmp) *p = (T)v; return o; }
+inline T atomicCAS(T *p, U cmp, W v) {
+    return emu_rmw(p, [&](T o) { return o == (T)cmp ? (T)v : o; });
+}
 
 // ---- runtime API ----------------------------------------------------------
 typedef int hipError_t;

@@ -221,6 +221,12 @@ def main() -> int:
     if name.endswith("_y"):  # preempt work-items at every third atomic (lost CAS races)
         os.environ["EMU_YIELD"] = "3"
         name = name[:-2]
+    elif name.endswith("_t"):  # workgroups of a launch on 4 host threads at once
+        os.environ["EMU_THREADS"] = "4"
+        name = name[:-2]
+    elif name.endswith("_r"):  # workgroups from the last one down
+        os.environ["EMU_ORDER"] = "reverse"
+        name = name[:-2]
     args = [int(a) for a in sys.argv[2:]]
     use_emulator()
     t.init(0)

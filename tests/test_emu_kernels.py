@@ -5,6 +5,9 @@ Each case is a child process (the ASan runtime must be preloaded); an
 out-of-bounds or use-after-free access in any kernel aborts it with ASan's
 report.  These cases need no GPU: they check memory safety and results of
 the same kernel sources that libtetra_hip.so is built from, at small sizes.
+Case-name suffixes: _y preempts work-items at atomics, _t runs the
+workgroups of each launch on four host threads at once, _r runs them in
+reverse order.
 """
 import os
 import subprocess
@@ -45,6 +48,10 @@ CASES = [
     ("window", 1, 100, 760, 16, 2), ("window", 3, 300, 2200, 40, 4),
     # TE, the vertex-lane hot path, then ET/TT/VF on the same mesh
     ("after_hot_path", 8, 5, 1), ("after_hot_path", 8, 5, 2),
+    # workgroups of each launch run concurrently on host threads (their
+    # global-memory accesses interleave), or in reverse order
+    ("after_hot_path_t", 8, 5, 1), ("after_hot_path_r", 8, 5, 1),
+    ("all_relations_t", 3, 4, 1), ("vv_cp_t", 5, 6, 2),
     # the no-allocation entry point: two calls, the second smaller
     ("noalloc", 6, 7, 4),
     ("split", 3, 4),
