@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cstring>
 #include <deque>
 #include <map>
 #include <memory>
@@ -228,18 +229,48 @@ std::vector<Request *> take_group(tg_server &s) {
     return g;
 }
 
+// Request inputs are staged in pinned host memory by the slot's own thread
+// (one reused buffer per slot): a copy from pageable memory goes through the
+// runtime's bounce buffer, one transfer at a time across all slots, while a
+// copy from pinned memory runs on the DMA engines, and the slots' host-side
+// copies proceed in parallel.  The device copies out of the buffer are done
+// once the group's classes are back on the host (same stream), or after the
+// sync on an error, before the slot stages its next group.
+thread_local Pinned<char> t_stage;
+
+bool stage_inputs(const std::vector<Request *> &reqs, int64_t T, int64_t V, int32_t *&cells,
+                  float *&sc) {
+    if (!t_stage.resize(16 * (size_t)T + 4 * (size_t)V)) return false;
+    cells = (int32_t *)t_stage.data();
+    sc = (float *)(t_stage.data() + 16 * (size_t)T);  // 16-byte aligned
+    int64_t c = 0, p = 0;
+    for (const Request *r : reqs) {
+        std::memcpy(cells + 4 * c, r->cells, 16 * (size_t)r->T);
+        if (r->scalars) std::memcpy(sc + p, r->scalars, 4 * (size_t)r->V);
+        c += r->T;
+        p += r->V;
+    }
+    return true;
+}
+
 int run_group(std::vector<Request *> &reqs, Group &g) {
     const auto t0 = Clock::now();
     for (Request *r : reqs) r->started = t0;
     int rc = TG_OK;
     Clock::time_point t1 = t0;
+    int32_t *cells = nullptr;
+    float *sc = nullptr;
     if (reqs.size() == 1) {
         Request &r = *reqs[0];
-        rc = tg_mesh_create(&g.mesh, r.cells, r.T, r.V, r.scalars, 0, 0, g.stream);
+        if (!stage_inputs(reqs, r.T, r.V, cells, sc)) rc = TG_ERR_ALLOC;
+        if (!rc)
+            rc = tg_mesh_create(&g.mesh, cells, r.T, r.V, r.scalars ? sc : nullptr, 0, 0,
+                                g.stream);
         t1 = Clock::now();
         if (!rc) rc = tg_vv_critpoints(g.mesh);
         if (!rc && !r.classes.resize(3 * (size_t)std::max<int64_t>(r.V, 1))) rc = TG_ERR_ALLOC;
         if (!rc) rc = tg_classes_copy(g.mesh, r.classes.data());
+        if (rc && g.mesh) (void)tg_sync(g.mesh);
     } else {
         const int32_t n = (int32_t)reqs.size();
         std::vector<int64_t> coff(n + 1, 0), poff(n + 1, 0);
@@ -247,14 +278,10 @@ int run_group(std::vector<Request *> &reqs, Group &g) {
             coff[i + 1] = coff[i] + reqs[i]->T;
             poff[i + 1] = poff[i] + reqs[i]->V;
         }
-        std::vector<int32_t> cells(4 * (size_t)coff[n]);
-        std::vector<float> sc((size_t)poff[n]);
-        for (int32_t i = 0; i < n; i++) {
-            std::copy(reqs[i]->cells, reqs[i]->cells + 4 * reqs[i]->T, cells.begin() + 4 * coff[i]);
-            std::copy(reqs[i]->scalars, reqs[i]->scalars + reqs[i]->V, sc.begin() + poff[i]);
-        }
-        rc = tg_batch_create(&g.batch, n, coff.data(), cells.data(), poff.data(), sc.data(), 0, 0,
-                             g.stream);
+        if (!stage_inputs(reqs, coff[n], poff[n], cells, sc)) rc = TG_ERR_ALLOC;
+        if (!rc)
+            rc = tg_batch_create(&g.batch, n, coff.data(), cells, poff.data(), sc, 0, 0,
+                                 g.stream);
         t1 = Clock::now();
         if (!rc) rc = tg_batch_vv_critpoints(g.batch);
         std::vector<uint32_t> all(3 * (size_t)std::max<int64_t>(poff[n], 1));
@@ -269,6 +296,7 @@ int run_group(std::vector<Request *> &reqs, Group &g) {
             std::copy(all.begin() + 3 * poff[i], all.begin() + 3 * poff[i] + k,
                       reqs[i]->classes.data());
         }
+        if (rc && g.batch) (void)tg_batch_sync(g.batch);
     }
     const auto t2 = Clock::now();
     for (Request *r : reqs) {
