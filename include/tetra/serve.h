@@ -61,7 +61,10 @@ typedef struct tg_request_result {
 
 int tg_server_create(tg_server **out, const tg_server_config *cfg);
 /* followups: TG_FLAG(TG_REL_VT) | TG_FLAG(TG_REL_TT) to compute with the
- * first result (counted in its latency), optionally | TG_SERVE_COPY. */
+ * first result (counted in its latency), optionally | TG_SERVE_COPY.  cells
+ * and scalars may be pageable; they are read (into the slot's pinned
+ * staging buffer) when the request starts, and must stay valid until
+ * tg_server_wait returns. */
 int tg_server_submit(tg_server *s, const int32_t *cells, int64_t n_cells, int64_t n_points,
                      const float *scalars, uint32_t followups, uint64_t *id);
 /* Block until the request's first result; pointers stay valid until release. */
