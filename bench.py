@@ -860,6 +860,11 @@ def main() -> int:
             # share of all SIMD cycles spent issuing VALU work (instruction
             # count from rocprofv3, duration live): the kernel's compute bound
             "valu_busy_frac": valu_busy(workload, dom_ms),
+            # the step's compulsory bytes priced against this kernel alone:
+            # the fraction the step could reach if every other launch took no
+            # time
+            "frac_if_alone": (round(B / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                              if dom_ms > 0 else None),
         },
         "stages_ms": {k: round(v, 4) for k, v in stages},
     }
