@@ -240,7 +240,7 @@ thread_local Pinned<char> t_stage;
 
 bool stage_inputs(const std::vector<Request *> &reqs, int64_t T, int64_t V, int32_t *&cells,
                   float *&sc) {
-    if (!t_stage.resize(16 * (size_t)T + 4 * (size_t)V)) return false;
+    if (!t_stage.resize(std::max<size_t>(16 * (size_t)T + 4 * (size_t)V, 16))) return false;
     cells = (int32_t *)t_stage.data();
     sc = (float *)(t_stage.data() + 16 * (size_t)T);  // 16-byte aligned
     int64_t c = 0, p = 0;
