@@ -103,6 +103,55 @@ def read_counters(path: Path) -> dict:
     return {k: {c: list(v.values()) for c, v in d.items()} for k, d in out.items()}
 
 
+def trace_summary(d: Path) -> int:
+    """Markdown summary of a rocprofv3 kernel trace in d, over the window after
+    the longest idle gap of at least half a second (the last phase of the run,
+    e.g. the serving loop after its meshes were generated on the host): how much of the window some
+    kernel was running (union of the kernel intervals), the mean number
+    running at once, and the kernels with the most summed time."""
+    import csv
+    import re
+
+    files = sorted(d.glob("**/*kernel_trace.csv"))
+    if not files:
+        raise SystemExit(f"no kernel trace under {d}")
+    rows = []
+    with open(files[0]) as fh:
+        for r in csv.DictReader(fh):
+            m = re.search(r"(k_[A-Za-z0-9_]+|__amd_rocclr_[A-Za-z]+)", r["Kernel_Name"])
+            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]),
+                         m.group(1) if m else r["Kernel_Name"][:48]))
+    rows.sort()
+    # busy segments (unions of overlapping kernels) and the longest gap between them
+    segs = []
+    for s, e, _ in rows:
+        if segs and s <= segs[-1][1]:
+            segs[-1][1] = max(segs[-1][1], e)
+        else:
+            segs.append([s, e])
+    gaps = [(segs[i + 1][0] - segs[i][1], i + 1) for i in range(len(segs) - 1)]
+    gap, first = max(gaps) if gaps else (0, 0)
+    if gap < 500_000_000:  # no idle phase of half a second: the whole trace
+        gap, first = 0, 0
+    t0 = segs[first][0]
+    win = [x for x in rows if x[0] >= t0]
+    busy = sum(e - s for s, e in segs[first:])
+    span = segs[-1][1] - t0
+    per = {}
+    for s, e, k in win:
+        n, tot = per.get(k, (0, 0))
+        per[k] = (n + 1, tot + e - s)
+    total = sum(e - s for s, e, _ in win)
+    print(f"window after the longest idle gap ({gap / 1e9:.3f} s): {len(win)} kernels over "
+          f"{span / 1e9:.3f} s; some kernel running {busy / 1e9:.3f} s ({busy / span:.1%} of "
+          f"the window); summed kernel time {total / 1e9:.3f} s ({total / busy:.2f} kernels "
+          f"at once while busy)\n")
+    print("| kernel | calls | summed ms | share | mean us |\n|---|---:|---:|---:|---:|")
+    for k, (n, tot) in sorted(per.items(), key=lambda x: -x[1][1])[:14]:
+        print(f"| {k} | {n} | {tot / 1e6:.1f} | {tot / total:.1%} | {tot / n / 1e3:.1f} |")
+    return 0
+
+
 def summarize(d: Path, workload: str) -> int:
     """Write profiles/*_<workload>.{json,md} from a directory of rocprofv3
     outputs made by the commands that the written files list:
@@ -740,12 +789,16 @@ def main() -> int:
     ap.add_argument("--hot-path-only", action="store_true",
                     help="only the benchmark mesh's warm-up, timed and staged steps (for "
                          "counter profiles: every dispatch is the hot path's)")
+    ap.add_argument("--trace-summary", type=Path, default=None, metavar="DIR",
+                    help="print the busy share and the top kernels of a rocprofv3 kernel trace in DIR")
     ap.add_argument("--summarize", type=Path, default=None, metavar="DIR",
                     help="only write profiles/*.json|md from the rocprofv3 outputs in DIR "
                          "(no GPU)")
     args = ap.parse_args()
     if args.summarize:
         return summarize(args.summarize, f"kuhn{args.n}_gaussians")
+    if args.trace_summary:
+        return trace_summary(args.trace_summary)
     if args.hot_path_only:
         args.no_cpu = args.no_relations = args.no_irregular = args.no_partitions = True
         args.batch_meshes = args.serve_requests = 0
