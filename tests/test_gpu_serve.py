@@ -120,3 +120,25 @@ def test_concurrent_submitters(gpu):
     for k, (cells, V, s) in enumerate(reqs):
         ref = O.critpoints(O.vv(cells, V), s.astype(np.float64), V)
         assert np.array_equal(out[k]["classes"], ref), f"request {k}"
+
+
+def test_degenerate_requests(gpu):
+    """A mesh with no cells, one cell, and one cell beside an isolated vertex,
+    served alone and fused into one group: classes equal the oracle's (a
+    vertex in no cell has no lower or upper link: class 4)."""
+    from tetragpu_amd.serve import Server
+    one = np.array([[0, 1, 2, 3]], np.int32)
+    reqs = [(np.zeros((0, 4), np.int32), 3, np.array([0.5, 0.25, 1.0], np.float32)),
+            (one, 4, np.array([0.0, 1.0, 2.0, 3.0], np.float32)),
+            (one, 5, np.array([3.0, 1.0, 2.0, 0.0, 9.0], np.float32))]
+    for cfg in (dict(slots=1, fuse_max_meshes=1), dict(slots=1, fuse_max_meshes=8)):
+        with Server(**cfg) as srv:
+            ids = [srv.submit(c, V, s) for c, V, s in reqs]
+            got = [srv.wait(i) for i in ids]
+            for i in ids:
+                srv.release(i)
+        for k, ((cells, V, s), r) in enumerate(zip(reqs, got)):
+            assert r["status"] == 0, (cfg, k)
+            ref = O.critpoints(O.vv(cells, V), s.astype(np.float64), V)
+            assert np.array_equal(r["classes"], ref), (cfg, k, r["classes"], ref)
+    assert ref[4].tolist() == [0, 0, 4]
